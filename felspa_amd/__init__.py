@@ -1,0 +1,12 @@
+"""felspa_amd: MI355X (gfx950) replacement for FELSPA's Q2-Q1 Stokes
+saddle-point solve (StokesLinearSystem::solve), behind a C-ABI
+(include/fs_stokes.h) with hand-written HIP kernels."""
+from . import _lib
+from .configs import CONFIGS, make_problem, solver_settings
+from .stokes import (FC, SCR, CompareTest, NoConvergence, SparseILU, SparseMatrix,
+                     StokesLinearSystem, StokesSolverControl, ilu_symbolic)
+from .system import StokesSystem, problem
+
+__all__ = ["StokesLinearSystem", "StokesSolverControl", "SparseILU", "SparseMatrix",
+           "NoConvergence", "StokesSystem", "problem", "CONFIGS", "make_problem",
+           "solver_settings", "ilu_symbolic", "FC", "SCR", "CompareTest", "_lib"]

@@ -1,0 +1,45 @@
+"""The five Stokes workloads (matrix/field/BC parameters and solver settings).
+
+C1  2D inclusion, 64x64 Q2-Q1, shear top/bottom + periodic x, ILU(0) (CPU-runnable)
+C2  2D Rayleigh-Taylor, 256x256, free slip, gravity, ILU(1)
+C3  3D porphyroclast, 32^3, linear simple shear, ILU(1)
+C4  2D 128x128 seeded ellipses (learned warm start), simple shear
+C5  3D 48^3 seeded ellipsoids, simple shear, ILU(2)
+Seeds follow seed = 1000 * config_id + sample_id.
+"""
+from . import _lib as L
+from .system import problem
+
+CONFIGS = {
+    "C1": dict(dim=2, n_cells=64, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_TOPBOTTOM_SHEAR_PERIODIC,
+               eta_matrix=1.0, eta_incl=1e3, radius=0.1, ilu_k_A=0, ilu_k_S=0, hint_nodes=5),
+    "C2": dict(dim=2, n_cells=256, field=L.FS_FIELD_LAYERS, bc=L.FS_BC_FREE_SLIP,
+               eta_matrix=1.0, eta_incl=100.0, rho_matrix=1.0, rho_incl=1.1, amp=0.02,
+               gravity=1.0, ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+    "C3": dict(dim=3, n_cells=32, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_LINEAR_SHEAR,
+               eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+    "C4": dict(dim=2, n_cells=128, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
+               eta_matrix=1.0, n_ellipses=(1, 8), axes=(0.05, 0.2), logc=(0.0, 3.0),
+               ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+    "C5": dict(dim=3, n_cells=48, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
+               eta_matrix=1.0, n_ellipses=(1, 4), axes=(0.08, 0.25), logc=(1.0, 3.0),
+               ilu_k_A=2, ilu_k_S=0, hint_nodes=7),
+}
+
+_SOLVER_KEYS = ("ilu_k_A", "ilu_k_S", "hint_nodes")
+
+
+def make_problem(name, n_cells=None, sample=0):
+    c = dict(CONFIGS[name])
+    for k in _SOLVER_KEYS:
+        c.pop(k, None)
+    if n_cells is not None:
+        c["n_cells"] = n_cells
+    cid = int(name[1:])
+    c["seed"] = 1000 * cid + sample
+    return problem(**c)
+
+
+def solver_settings(name):
+    c = CONFIGS[name]
+    return {k: c[k] for k in _SOLVER_KEYS}

@@ -1,0 +1,370 @@
+// MI355X Stokes saddle-point solver: device data, setup, Krylov drivers and
+// the C-ABI of include/fs_stokes.h.
+//
+// Internally every vector lives in the solver's order: velocity rows permuted
+// by the A-block ILU schedule, pressure rows by the M_p ILU schedule, so the
+// preconditioner never gathers. The caller's order is restored at the
+// boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fs_common.h"
+#include "fs_stokes.h"
+#include "kernels.h"
+#include "ordering.h"
+#include "solver_internal.h"
+
+using namespace fsk;
+
+namespace fs {
+
+// ---------------------------------------------------------------------------
+// errors and device memory
+static int64_t g_mem_cap = 0;
+static int64_t g_mem_used = 0;
+}  // namespace fs
+
+extern "C" void fs_set_device_memory_cap(int64_t bytes) { fs::g_mem_cap = bytes; }
+
+namespace fs {
+
+FsError::FsError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+      throw FsError(FS_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+    throw FsError(FS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+void* dev_alloc(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (g_mem_cap > 0 && g_mem_used + (int64_t)bytes > g_mem_cap)
+    throw FsError(FS_ERR_OUT_OF_MEMORY, "device memory cap exceeded");
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  g_mem_used += (int64_t)bytes;
+  return p;
+}
+
+DevBuf::~DevBuf() { reset(); }
+void DevBuf::reset() {
+  if (p) {
+    (void)hipFree(p);
+    g_mem_used -= (int64_t)bytes;
+  }
+  p = nullptr;
+  bytes = 0;
+}
+void DevBuf::alloc(size_t b) {
+  reset();
+  p = dev_alloc(b);
+  bytes = b;
+}
+void DevBuf::upload(const void* src, size_t b) {
+  if (bytes < b) alloc(b);
+  hip_check(hipMemcpy(p, src, b, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+void ensure_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw FsError(FS_ERR_NO_DEVICE, "no HIP device visible");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    throw FsError(FS_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library built for gfx950");
+}
+
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+uint64_t timeout_ticks() {
+  static uint64_t t = 0;
+  if (!t) {
+    int dev = 0, khz = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    if (khz <= 0) khz = 100000;
+    t = (uint64_t)khz * 1000ull * 5ull;  // 5 s
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// host CSR input checks (caller layout; deal.II diagonal-first rows accepted)
+HostCsr csr_from_abi(const fs_csr* m, const char* name, bool need_vals) {
+  if (!m || !m->row_ptr || !m->col_idx || (need_vals && !m->val))
+    throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": null CSR array");
+  if (m->n_rows < 0 || m->n_cols < 0 || m->n_cols > INT32_MAX)
+    throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": bad dimensions");
+  HostCsr h;
+  h.n_rows = m->n_rows;
+  h.n_cols = m->n_cols;
+  const int64_t nnz = m->row_ptr[m->n_rows];
+  if (m->row_ptr[0] != 0 || nnz < 0) throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": bad row_ptr");
+  h.rp.assign(m->row_ptr, m->row_ptr + m->n_rows + 1);
+  h.ci.resize(nnz);
+  h.v.resize(need_vals || m->val ? nnz : 0);
+  std::vector<std::pair<int32_t, double>> tmp;
+  for (int64_t r = 0; r < h.n_rows; ++r) {
+    if (h.rp[r + 1] < h.rp[r]) throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": row_ptr not monotone");
+    tmp.clear();
+    for (int64_t p = h.rp[r]; p < h.rp[r + 1]; ++p) {
+      const int32_t c = m->col_idx[p];
+      if (c < 0 || c >= h.n_cols) throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": column out of range");
+      tmp.emplace_back(c, m->val ? m->val[p] : 0.0);
+    }
+    // deal.II rows store the diagonal first; everything is re-sorted here
+    std::sort(tmp.begin(), tmp.end(),
+              [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) { return a.first < b.first; });
+    for (size_t t = 1; t < tmp.size(); ++t)
+      if (tmp[t].first == tmp[t - 1].first) throw FsError(FS_ERR_BAD_INPUT, std::string(name) + ": duplicate column");
+    int64_t p = h.rp[r];
+    for (auto& e : tmp) {
+      h.ci[p] = e.first;
+      if (!h.v.empty()) h.v[p] = e.second;
+      ++p;
+    }
+  }
+  return h;
+}
+
+// position map: sorted-layout entry order of the caller's arrays. The caller
+// may pass diagonal-first rows; map_caller[e] = index into the sorted row.
+std::vector<int64_t> caller_to_sorted(const fs_csr* m) {
+  const int64_t nnz = m->row_ptr[m->n_rows];
+  std::vector<int64_t> map(nnz);
+  std::vector<std::pair<int32_t, int64_t>> tmp;
+  for (int64_t r = 0; r < m->n_rows; ++r) {
+    tmp.clear();
+    for (int64_t p = m->row_ptr[r]; p < m->row_ptr[r + 1]; ++p) tmp.emplace_back(m->col_idx[p], p);
+    std::sort(tmp.begin(), tmp.end());
+    int64_t q = m->row_ptr[r];
+    for (auto& e : tmp) map[e.second] = q++;
+  }
+  return map;
+}
+
+HostPattern pattern_of(const HostCsr& m) {
+  HostPattern p;
+  p.n = m.n_rows;
+  p.rp = m.rp;
+  p.ci = m.ci;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// ILU factor on the device
+void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max) {
+  sched = sc;
+  n = src_pattern_new.n;
+  HostPattern F = symbolic_ilu(src_pattern_new, k, &sched);
+  if (!schedule_is_valid(F, sched))
+    throw FsError(FS_ERR_UNSUPPORTED, "ILU fill joins two blocks of one stage; use a larger block size");
+  nnz = F.nnz();
+  std::vector<int32_t> ol(n), od(n), ou(n), lp(n), up(n);
+  int64_t sum_lext = 0, sum_uext = 0;
+  cap = 0;
+  max_len = 0;
+  for (int b = 0; b < sched.n_blocks(); ++b) {
+    const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+    int32_t lacc = 0, uacc = 0;
+    for (int32_t r = r0; r < r1; ++r) {
+      const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
+      const int32_t* c = F.ci.data() + base;
+      const int32_t* e = c + len;
+      const int32_t olint = (int32_t)(std::lower_bound(c, e, r0) - c);
+      const int32_t odiag = (int32_t)(std::lower_bound(c, e, r) - c);
+      const int32_t ouext = (int32_t)(std::lower_bound(c, e, r1) - c);
+      if (odiag >= len || c[odiag] != r) throw FsError(FS_ERR_BAD_INPUT, "ILU pattern lacks a diagonal");
+      ol[r] = olint;
+      od[r] = odiag;
+      ou[r] = ouext;
+      lp[r] = lacc;
+      up[r] = uacc;
+      lacc += odiag - olint;
+      uacc += ouext - odiag - 1;
+      sum_lext += olint;
+      sum_uext += len - ouext;
+      max_len = std::max<int>(max_len, (int)len);
+    }
+    cap = std::max<int>(cap, std::max(lacc, uacc));
+  }
+  max_rows = std::max(1, (int)sched.max_block_rows);
+  // LDS budget: 4 waves per workgroup, <= 64 KiB per workgroup
+  const int per_wave_budget = 16 * 1024;
+  const int fixed = (int)lds_bytes_per_wave(max_rows, 0, tsz_max);
+  const int cap_fit = std::max(0, (per_wave_budget - fixed) / (tsz_max + 2));
+  cap = std::min(cap, cap_fit);
+  if (fixed > per_wave_budget) throw FsError(FS_ERR_UNSUPPORTED, "ILU block too large for LDS");
+  const double avg = n ? (double)(sum_lext + sum_uext) / (2.0 * n) : 0;
+  G = avg > 48 ? 64 : avg > 24 ? 32 : avg > 12 ? 16 : avg > 6 ? 8 : 4;
+  rp.upload(F.rp.data(), 8 * (n + 1));
+  ci.upload(F.ci.data(), 4 * nnz);
+  v64.alloc(8 * nnz);
+  o_lint.upload(ol.data(), 4 * n);
+  o_diag.upload(od.data(), 4 * n);
+  o_uext.upload(ou.data(), 4 * n);
+  lpre.upload(lp.data(), 4 * n);
+  upre.upload(up.data(), 4 * n);
+  blk_ptr.upload(sched.blk_ptr.data(), 4 * sched.blk_ptr.size());
+  stage_ptr.upload(sched.stage_ptr.data(), 4 * sched.stage_ptr.size());
+  host_pattern = std::move(F);
+}
+
+template <typename T>
+IluView<T> IluDev::view() const {
+  IluView<T> v;
+  v.rp = (const int64_t*)rp.p;
+  v.ci = (const int32_t*)ci.p;
+  v.v = std::is_same<T, double>::value ? (T*)v64.p : (T*)v32.p;
+  v.o_lint = (const int32_t*)o_lint.p;
+  v.o_diag = (const int32_t*)o_diag.p;
+  v.o_uext = (const int32_t*)o_uext.p;
+  v.lpre = (const int32_t*)lpre.p;
+  v.upre = (const int32_t*)upre.p;
+  v.blk_ptr = (const int32_t*)blk_ptr.p;
+  return v;
+}
+template IluView<double> IluDev::view<double>() const;
+template IluView<float> IluDev::view<float>() const;
+
+static int grid_for(int64_t n, int per_block = BLOCK, int cap = 4096) {
+  int64_t g = (n + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit,
+                       hipStream_t st, int* d_err, Counters& cnt) {
+  IluView<double> V = view<double>();
+  const size_t lds = ((4 * (size_t)max_len + 15) & ~(size_t)15) + 8 * (size_t)max_len;
+  if (lds > 160 * 1024) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel");
+  for (int s = 0; s < sched.n_stages(); ++s) {
+    const int b0 = sched.stage_ptr[s], b1 = sched.stage_ptr[s + 1];
+    const int g = std::max(1, std::min(b1 - b0, 8192));
+    hipLaunchKernelGGL(k_ilu_numeric_stage, dim3(g), dim3(WAVE), lds, st, V, b0, b1, srp, sci, sv, col_limit,
+                       max_len, d_err);
+    cnt.launches++;
+  }
+  cnt.bytes += (double)nnz * 12.0 * 2.0;
+  hip_check(hipGetLastError(), "ilu numeric launch");
+  if (!v32.p) v32.alloc(4 * nnz);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz,
+                     (const double*)v64.p, (float*)v32.p);
+  cnt.launches++;
+}
+
+template <typename T, typename TB, typename TY, int G, bool UNIT, bool FWD>
+static void launch_stage(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
+  const size_t per_wave = lds_bytes_per_wave(F.max_rows, F.cap, sizeof(T));
+  const int nb = b1 - b0;
+  const int g = std::max(1, std::min((nb + 3) / 4, 16384));
+  hipLaunchKernelGGL((k_ilu_stage<T, TB, TY, G, UNIT, FWD>), dim3(g), dim3(BLOCK), per_wave * 4, st,
+                     F.view<T>(), b0, b1, b, y, F.max_rows, F.cap);
+}
+
+template <typename T, typename TB, typename TY, bool UNIT, bool FWD>
+static void launch_stage_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
+  switch (F.G) {
+    case 4: launch_stage<T, TB, TY, 4, UNIT, FWD>(F, b0, b1, b, y, st); break;
+    case 8: launch_stage<T, TB, TY, 8, UNIT, FWD>(F, b0, b1, b, y, st); break;
+    case 16: launch_stage<T, TB, TY, 16, UNIT, FWD>(F, b0, b1, b, y, st); break;
+    case 32: launch_stage<T, TB, TY, 32, UNIT, FWD>(F, b0, b1, b, y, st); break;
+    default: launch_stage<T, TB, TY, 64, UNIT, FWD>(F, b0, b1, b, y, st); break;
+  }
+}
+
+// x = (LU)^{-1} b : forward into tmp, backward into x (tmp may alias x)
+template <typename T, typename TB>
+void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
+  const int ns = sched.n_stages();
+  for (int s = 0; s < ns; ++s)
+    launch_stage_g<T, TB, T, true, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
+  for (int s = ns - 1; s >= 0; --s)
+    launch_stage_g<T, T, T, false, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
+  cnt.launches += 2 * ns;
+  cnt.bytes += (double)nnz * (sizeof(T) + 4) + 16.0 * (n + 1) + 3.0 * n * sizeof(T) + 20.0 * n;
+}
+template void IluDev::apply<double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
+template void IluDev::apply<float, double>(const double*, float*, float*, hipStream_t, Counters&) const;
+template void IluDev::apply<float, float>(const float*, float*, float*, hipStream_t, Counters&) const;
+
+// transposed solve with a transposed factor (lower part = U^T with diagonal,
+// upper part = L^T unit)
+template <typename T>
+void IluDev::apply_transposed(const T* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
+  const int ns = sched.n_stages();
+  for (int s = 0; s < ns; ++s)
+    launch_stage_g<T, T, T, false, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
+  for (int s = ns - 1; s >= 0; --s)
+    launch_stage_g<T, T, T, true, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
+  cnt.launches += 2 * ns;
+}
+template void IluDev::apply_transposed<double>(const double*, double*, double*, hipStream_t, Counters&) const;
+template void IluDev::apply_transposed<float>(const float*, float*, float*, hipStream_t, Counters&) const;
+
+// ---------------------------------------------------------------------------
+// SpMV launch on a CSR (rows [row_off, row_off + nrows))
+template <typename TV, typename TX, typename TY, typename TB, typename TACC, int MODE>
+void spmv_launch(int G, int64_t nrows, int64_t row_off, const int64_t* rp, const int32_t* ci, const TV* v,
+                 const TX* x, TY* y, const TB* b, int64_t col_limit, int64_t x_off, hipStream_t st) {
+  const int rows_per_block = BLOCK / G;
+  const int g = (int)std::max<int64_t>(1, (nrows + rows_per_block - 1) / rows_per_block);
+#define FS_SPMV_CASE(GG)                                                                                   \
+  case GG:                                                                                                 \
+    hipLaunchKernelGGL((k_spmv<TV, TX, TY, TB, TACC, GG, MODE>), dim3(g), dim3(BLOCK), 0, st, nrows, row_off, \
+                       rp, ci, v, x, y, b, col_limit, x_off);                                              \
+    break;
+  switch (G) {
+    FS_SPMV_CASE(4)
+    FS_SPMV_CASE(8)
+    FS_SPMV_CASE(16)
+    FS_SPMV_CASE(32)
+    default:
+      FS_SPMV_CASE(64)
+  }
+#undef FS_SPMV_CASE
+}
+
+int lanes_for(double avg_row) {
+  return avg_row > 96 ? 64 : avg_row > 48 ? 32 : avg_row > 20 ? 16 : avg_row > 8 ? 8 : 4;
+}
+
+void DevCsr::upload_pattern(const HostCsr& h) {
+  n_rows = h.n_rows;
+  n_cols = h.n_cols;
+  nnz = h.nnz();
+  rp.upload(h.rp.data(), 8 * (n_rows + 1));
+  ci.upload(h.ci.data(), 4 * std::max<int64_t>(nnz, 1));
+  G = lanes_for(n_rows ? (double)nnz / n_rows : 1);
+}
+
+}  // namespace fs
+
+// explicit instantiations used by solver_krylov.hip
+namespace fs {
+template void spmv_launch<double, double, double, double, double, 0>(int, int64_t, int64_t, const int64_t*, const int32_t*, const double*, const double*, double*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<double, double, double, double, double, 1>(int, int64_t, int64_t, const int64_t*, const int32_t*, const double*, const double*, double*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<double, double, double, double, double, 2>(int, int64_t, int64_t, const int64_t*, const int32_t*, const double*, const double*, double*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<float, float, double, double, float, 0>(int, int64_t, int64_t, const int64_t*, const int32_t*, const float*, const float*, double*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<float, float, float, double, float, 2>(int, int64_t, int64_t, const int64_t*, const int32_t*, const float*, const float*, float*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<float, double, double, double, double, 0>(int, int64_t, int64_t, const int64_t*, const int32_t*, const float*, const double*, double*, const double*, int64_t, int64_t, hipStream_t);
+template void spmv_launch<float, float, float, float, float, 0>(int, int64_t, int64_t, const int64_t*, const int32_t*, const float*, const float*, float*, const float*, int64_t, int64_t, hipStream_t);
+}  // namespace fs

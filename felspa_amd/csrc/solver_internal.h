@@ -1,0 +1,80 @@
+// Internal declarations shared by the solver translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "fs_common.h"
+#include "fs_stokes.h"
+#include "kernels.h"
+#include "ordering.h"
+
+namespace fs {
+
+struct FsError : std::runtime_error {
+  int code;
+  FsError(int c, const std::string& m);
+};
+
+void hip_check(hipError_t e, const char* what);
+void* dev_alloc(size_t bytes);
+void ensure_device();
+int device_cus();
+uint64_t timeout_ticks();
+int lanes_for(double avg_row);
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void reset();
+  void alloc(size_t b);
+  void upload(const void* src, size_t b);
+  template <typename T>
+  T* as() const { return (T*)p; }
+};
+
+struct Counters {
+  double bytes = 0;
+  int64_t launches = 0;
+};
+
+struct DevCsr {
+  int64_t n_rows = 0, n_cols = 0, nnz = 0;
+  int G = 16;
+  DevBuf rp, ci, v64, v32;
+  void upload_pattern(const HostCsr& h);
+};
+
+struct IluDev {
+  int64_t n = 0, nnz = 0;
+  Schedule sched;
+  HostPattern host_pattern;
+  DevBuf rp, ci, v64, v32, o_lint, o_diag, o_uext, lpre, upre, blk_ptr, stage_ptr;
+  int max_rows = 1, cap = 0, max_len = 1, G = 16;
+  void build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max);
+  template <typename T>
+  fsk::IluView<T> view() const;
+  void factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit, hipStream_t st,
+                 int* d_err, Counters& cnt);
+  template <typename T, typename TB>
+  void apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;
+  template <typename T>
+  void apply_transposed(const T* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;
+};
+
+template <typename TV, typename TX, typename TY, typename TB, typename TACC, int MODE>
+void spmv_launch(int G, int64_t nrows, int64_t row_off, const int64_t* rp, const int32_t* ci, const TV* v,
+                 const TX* x, TY* y, const TB* b, int64_t col_limit, int64_t x_off, hipStream_t st);
+
+HostCsr csr_from_abi(const fs_csr* m, const char* name, bool need_vals);
+std::vector<int64_t> caller_to_sorted(const fs_csr* m);
+HostPattern pattern_of(const HostCsr& m);
+
+}  // namespace fs

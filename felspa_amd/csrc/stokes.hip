@@ -1,0 +1,788 @@
+// fs_handle: the Stokes solve behind include/fs_stokes.h.
+//
+// Reference flow (src/pde/stokes_ilu.cc:442-561): optional diagonal scaling,
+// block preconditioner reinitialisation (numeric ILU of A and of M_p), GMRES
+// with the block-Schur preconditioner, unscaling. Two Krylov drivers:
+//   FS_METHOD_GMRES_REF  left-preconditioned restarted GMRES with
+//                        ReductionControl(rtol ||b||, reduce) on the
+//                        preconditioned residual (deal.II semantics);
+//   FS_METHOD_FGMRES     right-preconditioned flexible GMRES whose
+//                        preconditioner may run in FP32 (FS_PREC_MIXED); each
+//                        restart begins from the FP64 true residual, so the
+//                        outer loop is iterative refinement, and the solve
+//                        stops when ||b - K x|| <= rtol ||b|| in FP64.
+// Orthogonalisation is classical Gram-Schmidt with one re-orthogonalisation
+// (two multi-dot passes per step instead of j+1 dependent dot products).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fs_stokes.h"
+#include "solver_internal.h"
+
+using namespace fsk;
+using namespace fs;
+
+namespace {
+
+// device scalar slots
+constexpr int SC_H1 = 0;                    // [MAXM]
+constexpr int SC_H2 = SC_H1 + MAXM;         // [MAXM]
+constexpr int SC_HCOL = SC_H2 + MAXM;       // [MAXM+1]
+constexpr int SC_CS = SC_HCOL + MAXM + 1;   // [MAXM]
+constexpr int SC_SN = SC_CS + MAXM;         // [MAXM]
+constexpr int SC_GAM = SC_SN + MAXM;        // [MAXM+1]
+constexpr int SC_Y = SC_GAM + MAXM + 1;     // [MAXM]
+constexpr int SC_NRM = SC_Y + MAXM;         // 1
+constexpr int SC_EST = SC_NRM + 1;          // 1
+constexpr int SC_TMP = SC_EST + 1;          // 4
+constexpr int SC_H = SC_TMP + 4;            // [(MAXM+1) * MAXM]
+constexpr int SC_TOTAL = SC_H + (MAXM + 1) * MAXM;
+
+struct Ctrl {  // dealii::ReductionControl::check (documented semantics)
+  int max_steps = 100;
+  double tol = 0, reduce = 0, reduced_tol = 0;
+  int last_step = 0;
+  double last_value = 0;
+  enum State { ITERATE, SUCCESS, FAILURE };
+  State check(int step, double v) {
+    if (step == 0) reduced_tol = v * reduce;
+    last_step = step;
+    last_value = v;
+    if (v <= reduced_tol) return SUCCESS;
+    if (v <= tol) return SUCCESS;
+    if (step >= max_steps || std::isnan(v)) return FAILURE;
+    return ITERATE;
+  }
+};
+
+template <typename T>
+T* dptr(const DevBuf& b, int64_t off = 0) {
+  return (T*)b.p + off;
+}
+
+}  // namespace
+
+struct fs_handle {
+  fs_opts o{};
+  int64_t nu = 0, np = 0, n = 0, ld = 0;
+  hipStream_t st = nullptr;
+  DevCsr K, Mp;
+  std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp;  // caller entry -> K / Mp position
+  std::vector<double> hK, hMp;                          // host value staging
+  IluDev FA, FS;
+  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin;
+  double* h_pin = nullptr;  // pinned scalars
+  bool dirty = true;
+  Counters cnt;
+  // learned guess
+  fs_guess_model* model = nullptr;
+  std::vector<float> guess_field;
+  int guess_ncells = 0;
+  std::vector<int64_t> g_vnode, g_pnode;
+  std::vector<int8_t> g_vcomp;
+
+  ~fs_handle() {
+    if (h_pin) (void)hipHostFree(h_pin);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  double* S(int off) const { return dptr<double>(sc, off); }
+
+  void sync() { hip_check(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+  double read_scalar(int off) {
+    hip_check(hipMemcpyAsync(h_pin, S(off), 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+    sync();
+    return h_pin[0];
+  }
+
+  // ||v||_2 of an FP64 device vector into sc[off]
+  void norm(const double* v, int64_t len, int off) {
+    const int g = std::min<int>(RED_BLOCKS, (int)std::max<int64_t>(1, (len + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL((k_dot_partial<double>), dim3(g), dim3(BLOCK), 0, st, len, v, v, dptr<double>(partial));
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, st, g, dptr<double>(partial), S(off), 1);
+    cnt.launches += 2;
+    cnt.bytes += 8.0 * len;
+  }
+
+  // y = K x (FP64) or y = b - K x
+  void kmult64(const double* xv, double* y, const double* bv) {
+    if (bv)
+      spmv_launch<double, double, double, double, double, 1>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
+                                                             K.v64.as<double>(), xv, y, bv, 0, 0, st);
+    else
+      spmv_launch<double, double, double, double, double, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
+                                                             K.v64.as<double>(), xv, y, (const double*)nullptr,
+                                                             0, 0, st);
+    cnt.launches++;
+    cnt.bytes += K.nnz * 12.0 + 8.0 * (n + 1) + 8.0 * n * (bv ? 3 : 2);
+  }
+
+  // inner CG on M_p (solver order)
+  template <typename T>
+  void inner_cg(const T* bp, T* xp) {
+    InnerCgArgs<T> a;
+    a.n = np;
+    a.mrp = Mp.rp.as<int64_t>();
+    a.mci = Mp.ci.as<int32_t>();
+    a.mv = std::is_same<T, double>::value ? (const T*)Mp.v64.p : (const T*)Mp.v32.p;
+    a.F = FS.view<T>();
+    a.stage_ptr = FS.stage_ptr.as<int32_t>();
+    a.n_stages = FS.sched.n_stages();
+    a.max_rows = FS.max_rows;
+    a.cap = FS.cap;
+    a.b = bp;
+    a.x = xp;
+    T* w = (T*)cg.p;
+    const int64_t lp = (np + 63) & ~int64_t(63);
+    a.g = w;
+    a.h = w + lp;
+    a.d = w + 2 * lp;
+    a.q = w + 3 * lp;
+    a.partial = dptr<double>(partial);
+    const bool mixed = std::is_same<T, float>::value;
+    a.tol = mixed ? 0.0 : o.inner_cg_tol;
+    a.reduce = mixed ? o.inner_cg_tol_mixed : o.inner_cg_tol;
+    a.max_it = mixed ? (o.inner_cg_max_mixed > 0 ? o.inner_cg_max_mixed : (int)np)
+                     : (o.inner_cg_max > 0 ? o.inner_cg_max : (int)np);
+    a.iters_out = dptr<int>(flags, 1);
+    a.fail_out = dptr<int>(flags, 2);
+    a.gb = dptr<GridBarrier>(gbar);
+    a.timeout = timeout_ticks();
+    const size_t lds = lds_bytes_per_wave(FS.max_rows, FS.cap, sizeof(T)) * (BLOCK / WAVE);
+    void* fn;
+    const int gm = Mp.G <= 8 ? 8 : Mp.G <= 16 ? 16 : 32;
+    fn = pick_cg_kernel<T>(gm, FS.G);
+    int per_cu = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLOCK, lds), "occupancy");
+    int nwg = (int)std::min<int64_t>((int64_t)std::max(1, per_cu) * device_cus(),
+                                     std::max<int64_t>(1, (np + 1023) / 1024));
+    nwg = std::max(1, std::min(nwg, 256));
+    void* args[] = {&a};
+    hip_check(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(BLOCK), args, (unsigned)lds, st),
+              "inner CG cooperative launch");
+    cnt.launches++;
+    cnt.bytes += (Mp.nnz + FS.nnz) * (sizeof(T) + 4.0) + 8.0 * np * sizeof(T);
+  }
+
+  template <typename T>
+  static void* pick_cg_kernel(int gm, int gf) {
+#define FS_CG(GM, GF) \
+  if (gm == GM && gf == GF) return (void*)&k_inner_cg<T, GM, GF>;
+    FS_CG(8, 4) FS_CG(8, 8) FS_CG(8, 16) FS_CG(8, 32) FS_CG(8, 64)
+    FS_CG(16, 4) FS_CG(16, 8) FS_CG(16, 16) FS_CG(16, 32) FS_CG(16, 64)
+    FS_CG(32, 4) FS_CG(32, 8) FS_CG(32, 16) FS_CG(32, 32) FS_CG(32, 64)
+#undef FS_CG
+    return (void*)&k_inner_cg<T, 16, 16>;
+  }
+
+  // BlockSchurPreconditionerBase::vmult (linear_operators.implement.h:200-209):
+  // dst_u = ILU_A^{-1} src_u ; t = B dst_u - src_p ; dst_p = CG(M_p, ILU_S; t)
+  template <typename T>
+  void precond(const double* src, T* dst) {
+    T* y = (T*)ytmp.p;
+    FA.apply<T, double>(src, y, dst, st, cnt);
+    const T* kv = std::is_same<T, double>::value ? (const T*)K.v64.p : (const T*)K.v32.p;
+    spmv_launch<T, T, T, double, T, 2>(K.G, np, nu, K.rp.as<int64_t>(), K.ci.as<int32_t>(), kv, dst, (T*)tp.p,
+                                       src + nu, nu, 0, st);
+    cnt.launches++;
+    inner_cg<T>((const T*)tp.p, dst + nu);
+  }
+
+  // CGS2 orthogonalisation of V[j+1] against V[0..j], Givens update
+  void arnoldi_tail(int j) {
+    double* Vb = dptr<double>(V);
+    double* w = Vb + (int64_t)(j + 1) * ld;
+    const int m = j + 1;
+    const int g = std::min<int>(RED_BLOCKS, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+    double* part = dptr<double>(partial);
+    hipLaunchKernelGGL((k_mdot_partial<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
+    hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(BLOCK), 0, st, g, m, part, S(SC_H1), 0);
+    hipLaunchKernelGGL((k_gs_update<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, S(SC_H1), w,
+                       (double*)nullptr);
+    hipLaunchKernelGGL((k_mdot_partial<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
+    hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(BLOCK), 0, st, g, m, part, S(SC_H2), 0);
+    hipLaunchKernelGGL((k_gs_update<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, S(SC_H2), w, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, st, g, part, S(SC_NRM), 1);
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(1), 0, st, j, MAXM, S(SC_H1), S(SC_H2), S(SC_NRM), S(SC_HCOL),
+                       S(SC_H), S(SC_CS), S(SC_SN), S(SC_GAM), S(SC_EST));
+    hipLaunchKernelGGL((k_scale_by_inv<double>), dim3(g), dim3(BLOCK), 0, st, n, w, w, S(SC_NRM), 0);
+    cnt.launches += 9;
+    cnt.bytes += 8.0 * n * (2.0 * (m + 1) + 2.0 * (m + 2) + 2.0);
+  }
+
+  void start_cycle(double* v0) {
+    const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL((k_scale_by_inv<double>), dim3(g), dim3(BLOCK), 0, st, n, v0, v0, S(SC_NRM), 0);
+    hipLaunchKernelGGL(k_init_gamma, dim3(1), dim3(1), 0, st, S(SC_GAM), S(SC_NRM), 0, S(SC_EST));
+    cnt.launches += 2;
+  }
+
+  template <typename TZ>
+  void finish_cycle(int dim, const TZ* Zb) {
+    const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(1), 0, st, dim, MAXM, S(SC_H), S(SC_GAM), S(SC_Y));
+    hipLaunchKernelGGL((k_update_x<TZ>), dim3(g), dim3(BLOCK), 0, st, n, dim, Zb, ld, S(SC_Y),
+                       dptr<double>(x));
+    cnt.launches += 2;
+    cnt.bytes += (double)n * (16.0 + dim * sizeof(TZ));
+  }
+
+  // right-preconditioned flexible GMRES, FP64 true residual at each restart
+  template <typename T>
+  void fgmres(double tol_abs, int max_steps, fs_stats& s) {
+    const int m = o.restart;
+    double* Vb = dptr<double>(V);
+    T* Zb = (T*)Z.p;
+    int acc = 0, cycles = 0;
+    bool conv = false;
+    double est = 0;
+    for (;;) {
+      kmult64(dptr<double>(x), Vb, dptr<double>(b));  // V0 = b - K x
+      norm(Vb, n, SC_NRM);
+      const double beta = read_scalar(SC_NRM);
+      est = beta;
+      if (cycles == 0) s.initial_rel_residual = beta;
+      if (!(beta > tol_abs)) { conv = std::isfinite(beta); break; }
+      if (acc >= max_steps) break;
+      start_cycle(Vb);
+      int dim = 0;
+      for (int j = 0; j < m && acc < max_steps; ++j) {
+        ++acc;
+        T* zj = Zb + (int64_t)j * ld;
+        precond<T>(Vb + (int64_t)j * ld, zj);
+        double* w = Vb + (int64_t)(j + 1) * ld;
+        if (std::is_same<T, double>::value)
+          spmv_launch<double, double, double, double, double, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
+                                                                 K.v64.as<double>(), (const double*)zj, w,
+                                                                 (const double*)nullptr, 0, 0, st);
+        else
+          spmv_launch<float, float, double, double, float, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
+                                                              K.v32.as<float>(), (const float*)zj, w,
+                                                              (const double*)nullptr, 0, 0, st);
+        cnt.launches++;
+        cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
+        arnoldi_tail(j);
+        dim = j + 1;
+        est = read_scalar(SC_EST);
+        if (!(est > tol_abs) || !std::isfinite(est)) break;
+      }
+      finish_cycle<T>(dim, Zb);
+      ++cycles;
+      if (!std::isfinite(est)) break;
+    }
+    s.outer_iters = acc;
+    s.restarts = cycles;
+    s.last_value = est;
+    s.status = conv ? FS_OK : FS_NOT_CONVERGED;
+  }
+
+  // deal.II SolverGMRES, left preconditioning (stokes_ilu.cc:536-542)
+  void gmres_ref(double bnorm, int max_steps, fs_stats& s) {
+    const int m = o.restart;
+    double* Vb = dptr<double>(V);
+    double* p = dptr<double>(r);
+    Ctrl c;
+    c.tol = o.rtol * bnorm;
+    c.reduce = o.reduce;
+    c.max_steps = max_steps;
+    int acc = 0, cycles = 0;
+    Ctrl::State stt = Ctrl::ITERATE;
+    do {
+      kmult64(dptr<double>(x), p, dptr<double>(b));
+      precond<double>(p, Vb);
+      norm(Vb, n, SC_NRM);
+      const double rho = read_scalar(SC_NRM);
+      stt = c.check(acc, rho);
+      if (stt != Ctrl::ITERATE) break;
+      start_cycle(Vb);
+      int dim = 0;
+      for (int j = 0; j < m && stt == Ctrl::ITERATE; ++j) {
+        ++acc;
+        kmult64(Vb + (int64_t)j * ld, p, nullptr);
+        precond<double>(p, Vb + (int64_t)(j + 1) * ld);
+        arnoldi_tail(j);
+        dim = j + 1;
+        stt = c.check(acc, read_scalar(SC_EST));
+      }
+      finish_cycle<double>(dim, Vb);
+      ++cycles;
+    } while (stt == Ctrl::ITERATE);
+    s.outer_iters = c.last_step;
+    s.restarts = cycles;
+    s.last_value = c.last_value;
+    s.status = stt == Ctrl::SUCCESS ? FS_OK : FS_NOT_CONVERGED;
+  }
+
+  void numeric_factorize() {
+    int* ferr = dptr<int>(flags, 0);
+    hip_check(hipMemsetAsync(ferr, 0, 4, st), "memset");
+    FA.factorize(K.rp.as<int64_t>(), K.ci.as<int32_t>(), K.v64.as<double>(), nu, st, ferr, cnt);
+    FS.factorize(Mp.rp.as<int64_t>(), Mp.ci.as<int32_t>(), Mp.v64.as<double>(), np, st, ferr, cnt);
+    int e = 0;
+    hip_check(hipMemcpyAsync(&e, ferr, 4, hipMemcpyDeviceToHost, st), "D2H");
+    sync();
+    if (e) throw FsError(FS_ERR_ZERO_PIVOT, "ILU: zero or non-finite pivot");
+    dirty = false;
+  }
+
+  void upload_values() {
+    K.v64.upload(hK.data(), 8 * K.nnz);
+    if (!K.v32.p) K.v32.alloc(4 * std::max<int64_t>(1, K.nnz));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(BLOCK), 0, st, K.nnz, K.v64.as<double>(),
+                       K.v32.as<float>());
+    Mp.v64.upload(hMp.data(), 8 * Mp.nnz);
+    if (!Mp.v32.p) Mp.v32.alloc(4 * std::max<int64_t>(1, Mp.nnz));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(BLOCK), 0, st, Mp.nnz, Mp.v64.as<double>(),
+                       Mp.v32.as<float>());
+    sync();
+    dirty = true;
+  }
+
+  void set_values(const double* A, const double* Bt, const double* B, const double* K11, const double* M) {
+    for (size_t e = 0; e < mapA.size(); ++e) hK[mapA[e]] = A[e];
+    for (size_t e = 0; e < mapBt.size(); ++e) hK[mapBt[e]] = Bt[e];
+    for (size_t e = 0; e < mapB.size(); ++e) hK[mapB[e]] = B[e];
+    for (size_t e = 0; e < mapK11.size(); ++e) hK[mapK11[e]] = K11[e];
+    for (size_t e = 0; e < mapMp.size(); ++e) hMp[mapMp[e]] = M[e];
+    upload_values();
+  }
+
+  // rhs, x: device FP64 in caller order
+  int solve_device(const double* d_rhs, double* d_x, fs_stats& s);
+};
+
+// ---------------------------------------------------------------------------
+namespace {
+
+// Build K in solver order from the caller's blocks; maps[e] = K position of
+// caller entry e of each block.
+void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, const fs_csr* K11) {
+  const auto& pa = h.FA.sched.perm;
+  const auto& ipa = h.FA.sched.iperm;
+  const auto& ps = h.FS.sched.perm;
+  const auto& ips = h.FS.sched.iperm;
+  const int64_t nu = h.nu, np = h.np, n = h.n;
+  HostCsr Kh;
+  Kh.n_rows = Kh.n_cols = n;
+  Kh.rp.assign(n + 1, 0);
+  auto rowlen = [](const fs_csr* m, int64_t r) { return m->row_ptr[r + 1] - m->row_ptr[r]; };
+  for (int64_t r = 0; r < nu; ++r) Kh.rp[r + 1] = rowlen(A, pa[r]) + rowlen(Bt, pa[r]);
+  for (int64_t q = 0; q < np; ++q) Kh.rp[nu + q + 1] = rowlen(B, ps[q]) + rowlen(K11, ps[q]);
+  for (int64_t r = 0; r < n; ++r) Kh.rp[r + 1] += Kh.rp[r];
+  Kh.ci.resize(Kh.rp[n]);
+  h.mapA.assign(A->row_ptr[nu], -1);
+  h.mapBt.assign(Bt->row_ptr[nu], -1);
+  h.mapB.assign(B->row_ptr[np], -1);
+  h.mapK11.assign(K11->row_ptr[np], -1);
+#pragma omp parallel
+  {
+    std::vector<std::pair<int32_t, int64_t>> tmp;  // (new col, tagged caller entry)
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t r = 0; r < n; ++r) {
+      tmp.clear();
+      const bool vel = r < nu;
+      const fs_csr* m1 = vel ? A : B;
+      const fs_csr* m2 = vel ? Bt : K11;
+      const int64_t o = vel ? pa[r] : ps[r - nu];
+      for (int64_t p = m1->row_ptr[o]; p < m1->row_ptr[o + 1]; ++p) tmp.emplace_back(ipa[m1->col_idx[p]], p << 1);
+      for (int64_t p = m2->row_ptr[o]; p < m2->row_ptr[o + 1]; ++p)
+        tmp.emplace_back((int32_t)(nu + ips[m2->col_idx[p]]), (p << 1) | 1);
+      std::sort(tmp.begin(), tmp.end());
+      int64_t q = Kh.rp[r];
+      for (auto& e : tmp) {
+        Kh.ci[q] = e.first;
+        const int64_t src = e.second >> 1;
+        const bool second = e.second & 1;
+        if (vel) (second ? h.mapBt : h.mapA)[src] = q;
+        else (second ? h.mapK11 : h.mapB)[src] = q;
+        ++q;
+      }
+    }
+  }
+  h.K.upload_pattern(Kh);
+  h.hK.assign(Kh.nnz(), 0.0);
+}
+
+void fill_stats_err(fs_stats* s, int code) {
+  if (s) {
+    std::memset(s, 0, sizeof(*s));
+    s->status = code;
+  }
+}
+
+#define FS_GUARD_BEGIN try {
+#define FS_GUARD_END                                          \
+  }                                                           \
+  catch (const FsError& e) {                                  \
+    fs::set_error("%s", e.what());                            \
+    return e.code;                                            \
+  }                                                           \
+  catch (const std::bad_alloc&) {                             \
+    fs::set_error("host allocation failed");                  \
+    return FS_ERR_OUT_OF_MEMORY;                              \
+  }                                                           \
+  catch (const std::exception& e) {                           \
+    fs::set_error("%s", e.what());                            \
+    return FS_ERR_BAD_INPUT;                                  \
+  }
+
+}  // namespace
+
+
+int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
+  std::memset(&s, 0, sizeof(s));
+  cnt = Counters{};
+  if (o.apply_diagonal_scaling)
+    throw FsError(FS_ERR_UNSUPPORTED, "diagonal scaling is not implemented on the GPU path");
+  if (o.precision == FS_PREC_MIXED && o.method == FS_METHOD_GMRES_REF)
+    throw FsError(FS_ERR_UNSUPPORTED, "reference-mode GMRES runs in FP64 only");
+  hipEvent_t e0, e1, e2;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  hip_check(hipEventCreate(&e2), "event");
+  struct EvGuard {
+    hipEvent_t a, b, c;
+    ~EvGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); (void)hipEventDestroy(c); }
+  } eg{e0, e1, e2};
+  hip_check(hipEventRecord(e0, st), "event record");
+  // setup: numeric factorisation (the reference refactorises in every solve,
+  // BlockSchurPreconditioner::reinitialize, stokes_ilu.cc:27-37)
+  numeric_factorize();
+  hip_check(hipEventRecord(e1, st), "event record");
+  const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+  const int32_t* pm = perm_all.as<int32_t>();
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, st, n, pm, d_rhs, dptr<double>(b), 0);
+  if (o.guess == FS_GUESS_ZERO)
+    hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
+  else
+    hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, st, n, pm, d_x, dptr<double>(x), 0);
+  hip_check(hipMemsetAsync(flags.p, 0, 16, st), "memset");  // flags[1] CG iterations, [2] CG failure
+  cnt.launches += 2;
+  norm(dptr<double>(b), n, SC_TMP);
+  norm(dptr<double>(x), n, SC_TMP + 1);
+  hip_check(hipMemcpyAsync(h_pin, S(SC_TMP), 16, hipMemcpyDeviceToHost, st), "D2H");
+  sync();
+  const double bnorm = h_pin[0], xnorm = h_pin[1];
+  if (!std::isfinite(bnorm) || !std::isfinite(xnorm))
+    throw FsError(FS_ERR_BAD_INPUT, "rhs or initial guess contains NaN/Inf");
+  s.guess_used = o.guess == FS_GUESS_ZERO ? FS_GUESS_ZERO : FS_GUESS_USER;
+  const int max_steps = o.max_outer > 0 ? o.max_outer : (int)n;
+  if (bnorm == 0.0) {
+    hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
+    s.status = FS_OK;
+  } else if (o.method == FS_METHOD_GMRES_REF) {
+    gmres_ref(bnorm, max_steps, s);
+  } else if (o.method == FS_METHOD_FGMRES) {
+    if (o.precision == FS_PREC_MIXED) fgmres<float>(o.rtol * bnorm, max_steps, s);
+    else fgmres<double>(o.rtol * bnorm, max_steps, s);
+    if (s.initial_rel_residual > 0) s.initial_rel_residual /= bnorm;
+  } else {
+    throw FsError(FS_ERR_UNSUPPORTED, "solution method not implemented");
+  }
+  hip_check(hipEventRecord(e2, st), "event record");
+  // true residual, back to the caller's order
+  kmult64(dptr<double>(x), dptr<double>(r), dptr<double>(b));
+  norm(dptr<double>(r), n, SC_TMP + 2);
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, st, n, pm, dptr<double>(x), d_x, 1);
+  int fl[4] = {0, 0, 0, 0};
+  hip_check(hipMemcpyAsync(fl, flags.p, 16, hipMemcpyDeviceToHost, st), "D2H");
+  hip_check(hipMemcpyAsync(h_pin + 2, S(SC_TMP + 2), 8, hipMemcpyDeviceToHost, st), "D2H");
+  GridBarrier gbh;
+  hip_check(hipMemcpyAsync(&gbh, gbar.p, sizeof(gbh), hipMemcpyDeviceToHost, st), "D2H");
+  sync();
+  if (gbh.err) {
+    hip_check(hipMemset(gbar.p, 0, sizeof(GridBarrier)), "memset");
+    throw FsError(FS_ERR_TIMEOUT, "inner CG grid barrier timed out");
+  }
+  float ms_setup = 0, ms_kry = 0;
+  hip_check(hipEventElapsedTime(&ms_setup, e0, e1), "elapsed");
+  hip_check(hipEventElapsedTime(&ms_kry, e1, e2), "elapsed");
+  s.inner_iters = fl[1];
+  s.true_rel_residual = h_pin[2] / (bnorm > 0 ? bnorm : 1.0);
+  s.t_setup_ms = ms_setup;
+  s.t_krylov_ms = ms_kry;
+  s.t_total_ms = ms_setup + ms_kry;
+  s.bytes_algorithmic = cnt.bytes;
+  s.n_kernel_launches = cnt.launches;
+  if (fl[2] && s.status == FS_OK && o.method == FS_METHOD_GMRES_REF) s.status = FS_NOT_CONVERGED;
+  return s.status;
+}
+
+// ---------------------------------------------------------------------------
+static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, const fs_csr* K11,
+                          const fs_csr* Mp, const fs_opts* opts) {
+  ensure_device();
+  if (opts) h.o = *opts;
+  else fs_opts_default(&h.o);
+  auto& o = h.o;
+  if (o.restart < 1 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [1, 31]");
+  if (o.ilu_k_A < 0 || o.ilu_k_S < 0 || o.ilu_k_A > 20 || o.ilu_k_S > 20)
+    throw FsError(FS_ERR_BAD_INPUT, "level of fill out of range");
+  if (!(o.rtol > 0)) throw FsError(FS_ERR_BAD_INPUT, "rtol must be positive");
+  // shapes
+  HostCsr hA = csr_from_abi(A, "A", true), hBt = csr_from_abi(Bt, "Bt", true), hB = csr_from_abi(B, "B", true),
+          hK11 = csr_from_abi(K11, "K11", true), hM = csr_from_abi(Mp, "Mp", true);
+  h.nu = hA.n_rows;
+  h.np = hM.n_rows;
+  h.n = h.nu + h.np;
+  if (hA.n_cols != h.nu || hBt.n_rows != h.nu || hBt.n_cols != h.np || hB.n_rows != h.np || hB.n_cols != h.nu ||
+      hK11.n_rows != h.np || hK11.n_cols != h.np || hM.n_cols != h.np)
+    throw FsError(FS_ERR_BAD_INPUT, "block shapes do not match");
+  if (h.n >= INT32_MAX) throw FsError(FS_ERR_BAD_INPUT, "system too large");
+  // orderings
+  const int bmax = 256;
+  HostPattern pA = pattern_of(hA), pM = pattern_of(hM);
+  Schedule sA, sS;
+  if (o.ordering == FS_ORDER_REFERENCE) {
+    sA = schedule_reference(h.nu, 128);
+    sS = schedule_reference(h.np, 128);
+  } else {
+    const int br = o.block_rows > 0 ? o.block_rows : 64;
+    sA = schedule_block_color(pA, o.ilu_k_A, br, o.block_hint, bmax);
+    sS = merge_stage_blocks(schedule_block_color(pM, o.ilu_k_S, 1, nullptr, bmax), 64);
+  }
+  // permuted patterns for the symbolic phase
+  HostCsr Anew = permute_square(hA, sA.perm, sA.iperm, nullptr);
+  std::vector<int64_t> mapM;
+  HostCsr Mnew = permute_square(hM, sS.perm, sS.iperm, &mapM);
+  h.FA.build(pattern_of(Anew), sA, o.ilu_k_A, 8);
+  h.FS.build(pattern_of(Mnew), sS, o.ilu_k_S, 8);
+  // K in solver order + value maps from the caller's layout
+  build_K(h, A, Bt, B, K11);
+  h.Mp.upload_pattern(Mnew);
+  // mapM is indexed by the sorted-layout entries of hM; convert to caller entries
+  {
+    std::vector<int64_t> c2s = caller_to_sorted(Mp);
+    h.mapMp.resize(c2s.size());
+    for (size_t e = 0; e < c2s.size(); ++e) h.mapMp[e] = mapM[c2s[e]];
+  }
+  h.hMp.assign(h.Mp.nnz, 0.0);
+  // device vectors
+  h.ld = (h.n + 63) & ~int64_t(63);
+  const int m = o.restart;
+  h.b.alloc(8 * h.ld);
+  h.x.alloc(8 * h.ld);
+  h.r.alloc(8 * h.ld);
+  h.V.alloc(8 * h.ld * (m + 1));
+  h.Z.alloc((o.precision == FS_PREC_MIXED ? 4 : 8) * h.ld * m);
+  h.ytmp.alloc(8 * h.ld);
+  h.tp.alloc(8 * h.ld);
+  h.cg.alloc(8 * 4 * ((h.np + 63) & ~int64_t(63)));
+  h.vin.alloc(8 * h.ld);
+  h.partial.alloc(8 * RED_BLOCKS * MAXM);
+  h.sc.alloc(8 * SC_TOTAL);
+  h.gbar.alloc(sizeof(GridBarrier));
+  h.flags.alloc(64);
+  hip_check(hipMemset(h.gbar.p, 0, sizeof(GridBarrier)), "memset");
+  hip_check(hipMemset(h.sc.p, 0, 8 * SC_TOTAL), "memset");
+  hip_check(hipMemset(h.V.p, 0, 8 * h.ld * (m + 1)), "memset");
+  {
+    std::vector<int32_t> pa(h.n);
+    for (int64_t i = 0; i < h.nu; ++i) pa[i] = h.FA.sched.perm[i];
+    for (int64_t q = 0; q < h.np; ++q) pa[h.nu + q] = (int32_t)(h.nu + h.FS.sched.perm[q]);
+    h.perm_all.upload(pa.data(), 4 * h.n);
+  }
+  hip_check(hipStreamCreateWithFlags(&h.st, hipStreamNonBlocking), "stream");
+  hip_check(hipHostMalloc((void**)&h.h_pin, 64, hipHostMallocDefault), "hipHostMalloc");
+  h.set_values(A->val, Bt->val, B->val, K11->val, Mp->val);
+}
+
+extern "C" {
+
+int fs_stokes_create(const fs_csr* A, const fs_csr* Bt, const fs_csr* B, const fs_csr* K11, const fs_csr* Mp,
+                     const fs_opts* opts, fs_handle** out) {
+  if (!out) { fs::set_error("fs_stokes_create: out is null"); return FS_ERR_BAD_INPUT; }
+  *out = nullptr;
+  FS_GUARD_BEGIN
+  std::unique_ptr<fs_handle> h(new fs_handle());
+  create_handle(*h, A, Bt, B, K11, Mp, opts);
+  *out = h.release();
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_refactor(fs_handle* h, const double* A_val, const double* Bt_val, const double* B_val,
+                       const double* K11_val, const double* Mp_val) {
+  if (!h || !A_val || !Bt_val || !B_val || !K11_val || !Mp_val) {
+    fs::set_error("fs_stokes_refactor: null argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  FS_GUARD_BEGIN
+  h->set_values(A_val, Bt_val, B_val, K11_val, Mp_val);
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_solve_device(fs_handle* h, const double* d_rhs, double* d_x, fs_stats* stats) {
+  fs_stats tmp;
+  if (!stats) stats = &tmp;
+  if (!h || !d_rhs || !d_x) {
+    fs::set_error("fs_stokes_solve_device: null argument");
+    fill_stats_err(stats, FS_ERR_BAD_INPUT);
+    return FS_ERR_BAD_INPUT;
+  }
+  try {
+    return h->solve_device(d_rhs, d_x, *stats);
+  } catch (const FsError& e) {
+    fs::set_error("%s", e.what());
+    fill_stats_err(stats, e.code);
+    return e.code;
+  } catch (const std::exception& e) {
+    fs::set_error("%s", e.what());
+    fill_stats_err(stats, FS_ERR_BAD_INPUT);
+    return FS_ERR_BAD_INPUT;
+  }
+}
+
+int fs_stokes_solve(fs_handle* h, const double* rhs, double* x, fs_stats* stats) {
+  fs_stats tmp;
+  if (!stats) stats = &tmp;
+  if (!h || !rhs || !x) {
+    fs::set_error("fs_stokes_solve: null argument");
+    fill_stats_err(stats, FS_ERR_BAD_INPUT);
+    return FS_ERR_BAD_INPUT;
+  }
+  try {
+    const int64_t n = h->n;
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isfinite(rhs[i]) || (h->o.guess == FS_GUESS_USER && !std::isfinite(x[i])))
+        throw FsError(FS_ERR_BAD_INPUT, "rhs or initial guess contains NaN/Inf");
+    DevBuf drhs, dx;
+    drhs.upload(rhs, 8 * n);
+    dx.upload(x, 8 * n);
+    int rc = h->solve_device(drhs.as<double>(), dx.as<double>(), *stats);
+    hip_check(hipMemcpy(x, dx.p, 8 * n, hipMemcpyDeviceToHost), "D2H");
+    return rc;
+  } catch (const FsError& e) {
+    fs::set_error("%s", e.what());
+    fill_stats_err(stats, e.code);
+    return e.code;
+  } catch (const std::exception& e) {
+    fs::set_error("%s", e.what());
+    fill_stats_err(stats, FS_ERR_BAD_INPUT);
+    return FS_ERR_BAD_INPUT;
+  }
+}
+
+int fs_stokes_vmult(fs_handle* h, const double* src, double* dst, int32_t precision) {
+  if (!h || !src || !dst) { fs::set_error("fs_stokes_vmult: null argument"); return FS_ERR_BAD_INPUT; }
+  FS_GUARD_BEGIN
+  const int64_t n = h->n;
+  const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+  h->vin.upload(src, 8 * n);
+  const int32_t* pm = h->perm_all.as<int32_t>();
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->vin.as<double>(),
+                     h->b.as<double>(), 0);
+  if (precision == FS_PREC_MIXED) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, n, h->b.as<double>(),
+                       (float*)h->ytmp.p);
+    spmv_launch<float, float, double, double, float, 0>(h->K.G, n, 0, h->K.rp.as<int64_t>(), h->K.ci.as<int32_t>(),
+                                                        h->K.v32.as<float>(), (const float*)h->ytmp.p,
+                                                        h->r.as<double>(), (const double*)nullptr, 0, 0, h->st);
+  } else {
+    h->kmult64(h->b.as<double>(), h->r.as<double>(), nullptr);
+  }
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->r.as<double>(),
+                     h->vin.as<double>(), 1);
+  hip_check(hipMemcpyAsync(dst, h->vin.p, 8 * n, hipMemcpyDeviceToHost, h->st), "D2H");
+  h->sync();
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_precond_vmult(fs_handle* h, const double* src, double* dst, int32_t precision, fs_stats* stats) {
+  if (!h || !src || !dst) { fs::set_error("fs_stokes_precond_vmult: null argument"); return FS_ERR_BAD_INPUT; }
+  FS_GUARD_BEGIN
+  const int64_t n = h->n;
+  const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+  if (h->dirty) h->numeric_factorize();
+  h->cnt = Counters{};
+  hip_check(hipMemsetAsync(h->flags.p, 0, 16, h->st), "memset");
+  h->vin.upload(src, 8 * n);
+  const int32_t* pm = h->perm_all.as<int32_t>();
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->vin.as<double>(),
+                     h->b.as<double>(), 0);
+  if (precision == FS_PREC_MIXED) {
+    h->precond<float>(h->b.as<double>(), (float*)h->Z.p);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(BLOCK), 0, h->st, n, (const float*)h->Z.p,
+                       h->r.as<double>());
+  } else {
+    h->precond<double>(h->b.as<double>(), h->r.as<double>());
+  }
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->r.as<double>(),
+                     h->vin.as<double>(), 1);
+  hip_check(hipMemcpyAsync(dst, h->vin.p, 8 * n, hipMemcpyDeviceToHost, h->st), "D2H");
+  int fl[4];
+  hip_check(hipMemcpyAsync(fl, h->flags.p, 16, hipMemcpyDeviceToHost, h->st), "D2H");
+  GridBarrier gbh;
+  hip_check(hipMemcpyAsync(&gbh, h->gbar.p, sizeof(gbh), hipMemcpyDeviceToHost, h->st), "D2H");
+  h->sync();
+  if (gbh.err) {
+    hip_check(hipMemset(h->gbar.p, 0, sizeof(GridBarrier)), "memset");
+    throw FsError(FS_ERR_TIMEOUT, "inner CG grid barrier timed out");
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->inner_iters = fl[1];
+    stats->status = fl[2] ? FS_NOT_CONVERGED : FS_OK;
+    stats->n_kernel_launches = h->cnt.launches;
+    stats->bytes_algorithmic = h->cnt.bytes;
+  }
+  return fl[2] ? FS_NOT_CONVERGED : FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* n_stages) {
+  if (!h) { fs::set_error("fs_stokes_get_ordering: null handle"); return FS_ERR_BAD_INPUT; }
+  const Schedule& s = which == 0 ? h->FA.sched : h->FS.sched;
+  if (perm) std::copy(s.perm.begin(), s.perm.end(), perm);
+  if (n_stages) *n_stages = s.n_stages();
+  return FS_OK;
+}
+
+int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model*, const float*, int32_t, const int64_t*,
+                              const int8_t*, const int64_t*) {
+  (void)h;
+  fs::set_error("fs_stokes_set_guess_field: learned guess not available in this build");
+  return FS_ERR_UNSUPPORTED;
+}
+
+void fs_stokes_destroy(fs_handle* h) { delete h; }
+
+}  // extern "C"
+
+extern "C" {
+
+void fs_opts_default(fs_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->method = FS_METHOD_FGMRES;
+  o->precision = FS_PREC_FP64;
+  o->ordering = FS_ORDER_BLOCK_COLOR;
+  o->ilu_k_A = 1;
+  o->ilu_k_S = 0;
+  o->restart = 23;
+  o->max_outer = 0;
+  o->rtol = 1e-8;
+  o->reduce = 1e-6;
+  o->inner_cg_max = 0;
+  o->inner_cg_tol = 1e-12;
+  o->inner_cg_tol_mixed = 1e-2;
+  o->inner_cg_max_mixed = 50;
+  o->apply_diagonal_scaling = 0;
+  o->guess = FS_GUESS_USER;
+  o->block_rows = 0;
+  o->block_hint = nullptr;
+  o->verbose = 0;
+}
+
+const char* fs_last_error(const fs_handle*) { return fs::get_error(); }
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+"""Q2-Q1 Stokes systems from the in-library generator (include/fs_assemble.h):
+blocks A, Bt, B, K11, Mp as scipy CSR, rhs, constraints and lattice maps."""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib as L
+
+BLOCK_NAMES = ("A", "Bt", "B", "K11", "Mp")
+
+
+def problem(dim, n_cells, field, bc, eta_matrix=1.0, eta_incl=1e3, rho_matrix=1.0, rho_incl=1.0,
+            radius=0.1, amp=0.02, gravity=0.0, shear_velocity=1.0, seed=0, n_ellipses=(1, 8),
+            axes=(0.05, 0.2), logc=(0.0, 3.0)):
+    return L.FsProblem(dim, n_cells, field, bc, eta_matrix, eta_incl, rho_matrix, rho_incl, radius,
+                       amp, gravity, shear_velocity, seed, n_ellipses[0], n_ellipses[1], axes[0],
+                       axes[1], logc[0], logc[1])
+
+
+class StokesSystem:
+    """Owns an fs_system; exposes copies of its arrays."""
+
+    def __init__(self, prob):
+        self._lib = L.lib()
+        self.h = C.c_void_p()
+        L.check(self._lib.fs_system_create(C.byref(prob), C.byref(self.h)), "fs_system_create")
+        info = L.FsSystemInfo()
+        L.check(self._lib.fs_system_get_info(self.h, C.byref(info)), "fs_system_get_info")
+        self.info = info
+        self.dim, self.n_cells = info.dim, info.n_cells
+        self.n_u, self.n_p = info.n_u, info.n_p
+        self.n = self.n_u + self.n_p
+        shapes = [(self.n_u, self.n_u), (self.n_u, self.n_p), (self.n_p, self.n_u),
+                  (self.n_p, self.n_p), (self.n_p, self.n_p)]
+        self.blocks = {}
+        for w, name in enumerate(BLOCK_NAMES):
+            rp, ci, v = L.P(L.i64)(), L.P(L.i32)(), L.P(L.f64)()
+            L.check(self._lib.fs_system_csr(self.h, w, C.byref(rp), C.byref(ci), C.byref(v)),
+                    "fs_system_csr")
+            m, n = shapes[w]
+            nnz = rp[m]
+            R = np.ctypeslib.as_array(rp, (m + 1,)).copy()
+            Ci = np.ctypeslib.as_array(ci, (max(nnz, 1),))[:nnz].copy()
+            Vv = np.ctypeslib.as_array(v, (max(nnz, 1),))[:nnz].copy()
+            self.blocks[name] = sp.csr_matrix((Vv, Ci, R), shape=(m, n))
+        self.rhs = np.ctypeslib.as_array(self._lib.fs_system_rhs(self.h), (self.n,)).copy()
+        k, ma, va = L.P(C.c_int8)(), L.P(L.i64)(), L.P(L.f64)()
+        self._lib.fs_system_constraints(self.h, C.byref(k), C.byref(ma), C.byref(va))
+        self.kind = np.ctypeslib.as_array(k, (self.n,)).copy()
+        self.master = np.ctypeslib.as_array(ma, (self.n,)).copy()
+        self.cval = np.ctypeslib.as_array(va, (self.n,)).copy()
+        vn, vc, pn = L.P(L.i64)(), L.P(C.c_int8)(), L.P(L.i64)()
+        self._lib.fs_system_dof_maps(self.h, C.byref(vn), C.byref(vc), C.byref(pn))
+        self.vnode = np.ctypeslib.as_array(vn, (self.n_u,)).copy()
+        self.vcomp = np.ctypeslib.as_array(vc, (self.n_u,)).copy()
+        self.pnode = np.ctypeslib.as_array(pn, (self.n_p,)).copy()
+        npn = (self.n_cells + 1) ** self.dim
+        self.log_eta = np.ctypeslib.as_array(self._lib.fs_system_log_eta_grid(self.h), (npn,)).copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.fs_system_destroy(self.h)
+            self.h = None
+
+    @property
+    def constrained(self):
+        return self.kind != L.FS_DOF_FREE
+
+    def K(self):
+        b = self.blocks
+        return sp.bmat([[b["A"], b["Bt"]], [b["B"], b["K11"]]]).tocsr()
+
+    def distribute(self, x):
+        """AffineConstraints::distribute: constrained entries from the constraints."""
+        x = np.array(x, dtype=np.float64)
+        self._lib.fs_system_distribute(self.h, x.ctypes.data_as(L.P(L.f64)))
+        return x
+
+    def block_hint(self, nodes_per_block):
+        out = np.zeros(self.n_u, np.int32)
+        L.check(self._lib.fs_system_block_hint(self.h, int(nodes_per_block),
+                                                out.ctypes.data_as(L.P(L.i32))),
+                "fs_system_block_hint")
+        return out

@@ -1,0 +1,186 @@
+/*
+ * fs_stokes.h -- C-ABI of the MI355X (gfx950) Stokes saddle-point solver.
+ *
+ * Drop-in for FELSPA's StokesLinearSystem<dim,double>::solve(soln, rhs, ctrl)
+ * (include/felspa/pde/stokes_ilu.h:232-236, src/pde/stokes_ilu.cc:442-561).
+ * The caller keeps the assembled blocks of the BlockSparseMatrix
+ *     K = [[A, B^T], [B, K11]],  preconditioner(1,1) = M_p(1/eta)
+ * in host memory as CSR arrays (deal.II's diagonal-first row layout is
+ * accepted as-is); the library copies them to the GPU once per sparsity
+ * pattern and owns every device buffer. No C++ types and no exceptions cross
+ * this boundary: every entry point returns an FS_* status code and
+ * fs_last_error() describes the last failure on the calling thread.
+ * Calls on one handle must be serialised by the caller (the reference
+ * solve() is not re-entrant either).
+ */
+#ifndef FS_STOKES_H
+#define FS_STOKES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  FS_OK = 0,
+  FS_NOT_CONVERGED = 1,      /* deal.II SolverControl::NoConvergence */
+  FS_ERR_BAD_INPUT = -1,     /* malformed CSR, size mismatch, NaN in rhs/guess */
+  FS_ERR_ZERO_PIVOT = -2,    /* ILU pivot zero or non-finite (DEBUG-only check in the reference, ilu.implement.h:498-499) */
+  FS_ERR_OUT_OF_MEMORY = -3, /* device or host allocation failed (or FS_MAX_DEVICE_BYTES exceeded) */
+  FS_ERR_HIP = -4,           /* HIP runtime error */
+  FS_ERR_UNSUPPORTED = -5,   /* option combination not implemented */
+  FS_ERR_TIMEOUT = -6,       /* a device-side bounded wait expired */
+  FS_ERR_NO_DEVICE = -7      /* no gfx950 device visible */
+};
+
+/* ---- options ----------------------------------------------------------- */
+enum {
+  FS_METHOD_GMRES_REF = 0, /* deal.II SolverGMRES semantics: left preconditioning,
+                              restart, ReductionControl(tol, reduce) on the
+                              preconditioned residual (stokes_ilu.cc:533-542) */
+  FS_METHOD_FGMRES = 1,    /* flexible right-preconditioned GMRES, stops on the
+                              FP64 true residual ||b-Kx|| <= rtol ||b|| */
+  FS_METHOD_SCR = 2        /* Schur-complement reduction (stokes_ilu.cc:564-710) */
+};
+enum { FS_PREC_FP64 = 0, FS_PREC_MIXED = 1 };
+enum {
+  FS_ORDER_REFERENCE = 0,   /* ILU on the caller's row order (sequential stages) */
+  FS_ORDER_BLOCK_COLOR = 1  /* ILU on a block-coloured symmetric permutation */
+};
+enum { FS_GUESS_USER = 0, FS_GUESS_ZERO = 1, FS_GUESS_LEARNED = 2 };
+
+typedef struct fs_csr {
+  int64_t n_rows, n_cols;
+  const int64_t* row_ptr; /* n_rows+1 */
+  const int32_t* col_idx; /* row_ptr[n_rows] */
+  const double* val;      /* row_ptr[n_rows]; may be NULL for pattern-only calls */
+  int32_t diag_first;     /* 1: square block stored with its diagonal first (deal.II) */
+} fs_csr;
+
+typedef struct fs_opts {
+  int32_t method;        /* FS_METHOD_* */
+  int32_t precision;     /* FS_PREC_* */
+  int32_t ordering;      /* FS_ORDER_* */
+  int32_t ilu_k_A;       /* Control::set_level_of_fill A (stokes_ilu.cc:139-147), default 1 */
+  int32_t ilu_k_S;       /* ... S, default 0 */
+  int32_t restart;       /* Krylov basis size; 23 = deal.II max_n_tmp_vectors 25 - 2 */
+  int32_t max_outer;     /* 0: n_u + n_p (stokes_ilu.cc:534) */
+  double rtol;           /* tol = rtol ||rhs|| (stokes_ilu.cc:533), default 1e-8 */
+  double reduce;         /* ReductionControl reduce (stokes_common.h:343-348), default 1e-6 */
+  int32_t inner_cg_max;  /* 0: n_p (linear_operators.implement.h:76) */
+  double inner_cg_tol;   /* InverseMatrixCG tol = reduce (linear_operators.h:44-45), default 1e-12 */
+  double inner_cg_tol_mixed; /* FP32 inner CG relative tolerance in FS_PREC_MIXED */
+  int32_t inner_cg_max_mixed;
+  int32_t apply_diagonal_scaling; /* StokesSolverControl::apply_diagonal_scaling (stokes_common.h:360) */
+  int32_t guess;         /* FS_GUESS_* */
+  int32_t block_rows;    /* target rows per block for FS_ORDER_BLOCK_COLOR (0: default) */
+  const int32_t* block_hint; /* optional [n_u] block id per velocity row (e.g. geometric boxes) */
+  int32_t verbose;
+} fs_opts;
+
+void fs_opts_default(fs_opts* opts);
+
+typedef struct fs_stats {
+  int32_t status;            /* FS_OK / FS_NOT_CONVERGED / error */
+  int32_t outer_iters;       /* Krylov iterations (SolverControl::last_step) */
+  int32_t restarts;          /* restart cycles */
+  int32_t guess_used;        /* FS_GUESS_* actually used (learned falls back to zero) */
+  int64_t inner_iters;       /* total inner CG iterations */
+  double last_value;         /* SolverControl::last_value: value seen by the stopping rule */
+  double initial_rel_residual; /* ||b - K x0|| / ||b|| */
+  double true_rel_residual;  /* ||b - K x|| / ||b|| in FP64 after the solve */
+  double t_setup_ms;         /* numeric factorisation + scaling */
+  double t_krylov_ms;        /* Krylov iterations */
+  double t_total_ms;
+  double bytes_algorithmic;  /* sum over kernels of their algorithmic HBM bytes */
+  int64_t n_kernel_launches;
+} fs_stats;
+
+typedef struct fs_handle fs_handle;
+
+/* StokesLinearSystem::setup_constraints_and_system + SparseILU symbolic phase
+ * (stokes_ilu.cc:226-257, ilu.implement.h:58-87): copies the blocks to the
+ * GPU, builds the orderings and the ILU(k) patterns of A and M_p. */
+int fs_stokes_create(const fs_csr* A, const fs_csr* Bt, const fs_csr* B, const fs_csr* K11,
+                     const fs_csr* Mp, const fs_opts* opts, fs_handle** out);
+
+/* New values on the same patterns (use_previous_sparsity = true,
+ * stokes_ilu.cc:547-548): numeric factorisation happens in the next solve. */
+int fs_stokes_refactor(fs_handle* h, const double* A_val, const double* Bt_val,
+                       const double* B_val, const double* K11_val, const double* Mp_val);
+
+/* StokesLinearSystem::solve (stokes_ilu.cc:442-468 -> solve_gmres :509-561):
+ * rhs [n_u+n_p] host, x [n_u+n_p] host: guess in, solution out. Constrained
+ * entries of the guess are zeroed first (stokes_ilu.cc:523-524). Returns
+ * FS_OK, FS_NOT_CONVERGED or an error. */
+int fs_stokes_solve(fs_handle* h, const double* rhs, double* x, fs_stats* stats);
+
+/* Same, with rhs and x already resident in device memory (FP64). */
+int fs_stokes_solve_device(fs_handle* h, const double* d_rhs, double* d_x, fs_stats* stats);
+
+/* K x (BlockSparseMatrix::vmult), host vectors [n_u+n_p]. */
+int fs_stokes_vmult(fs_handle* h, const double* src, double* dst, int32_t precision);
+
+/* BlockSchurPreconditionerBase::vmult (linear_operators.implement.h:200-209):
+ * dst = P^{-1} src with the current factorisation, host vectors [n_u+n_p].
+ * The inner CG starts from zero. */
+int fs_stokes_precond_vmult(fs_handle* h, const double* src, double* dst, int32_t precision,
+                            fs_stats* stats);
+
+/* Set the warm-start field for FS_GUESS_LEARNED: log10(eta) on the pressure
+ * node lattice, (N+1)^dim values, x fastest, plus the lattice maps of the dofs
+ * (see fs_system_dof_maps in fs_assemble.h). The model comes from
+ * fs_guess_model_* in fs_guess.h. */
+struct fs_guess_model;
+int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const float* log_eta,
+                              int32_t n_cells, const int64_t* vnode, const int8_t* vcomp,
+                              const int64_t* pnode);
+
+/* Row permutation used by FS_ORDER_BLOCK_COLOR for the A (which=0) and
+ * M_p (which=1) blocks: perm[new] = old. */
+int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* n_stages);
+
+const char* fs_last_error(const fs_handle* h);
+void fs_stokes_destroy(fs_handle* h);
+
+/* ---- SparseILU (include/felspa/linear_algebra/ilu.h:25-238) ------------- */
+typedef struct fs_ilu fs_ilu;
+
+/* SparseILU::initialize symbolic phase (ilu.implement.h:29-87) on the GPU
+ * object; ordering as above (FS_ORDER_REFERENCE keeps the caller's order). */
+int fs_ilu_create(const fs_csr* A, int32_t level_of_fill, int32_t ordering, int32_t block_rows,
+                  const int32_t* block_hint, fs_ilu** out);
+/* numeric_factorize (ilu.implement.h:438-541) on the GPU, values of A in the
+ * caller's order/pattern. */
+int fs_ilu_factorize(fs_ilu* ilu, const double* A_val);
+/* vmult (ilu.implement.h:544-595) / Tvmult (:626-680), host vectors in the
+ * caller's order; precision FS_PREC_FP64 or FS_PREC_MIXED (FP32 factors). */
+int fs_ilu_vmult(fs_ilu* ilu, const double* src, double* dst, int32_t precision);
+int fs_ilu_Tvmult(fs_ilu* ilu, const double* src, double* dst, int32_t precision);
+/* Factor pattern and values in the factor's (permuted) order, rows sorted,
+ * diagonal in place. Pass NULL arrays to query n / nnz. */
+int fs_ilu_get_factors(fs_ilu* ilu, int64_t* n, int64_t* nnz, int64_t* row_ptr, int32_t* col_idx,
+                       double* val, int32_t* perm);
+void fs_ilu_destroy(fs_ilu* ilu);
+
+/* Host symbolic ILU(k) with the reference's level-of-fill rules
+ * (ilu.implement.h:90-435); returns a malloc'd pattern the caller frees with
+ * fs_free. */
+int fs_ilu_symbolic(const fs_csr* A, int32_t level_of_fill, int64_t** row_ptr, int32_t** col_idx);
+void fs_free(void* p);
+
+/* ---- SparseMatrix::vmult on the GPU (CSR SpMV) -------------------------- */
+typedef struct fs_matrix fs_matrix;
+int fs_matrix_create(const fs_csr* M, fs_matrix** out);
+int fs_matrix_vmult(fs_matrix* M, const double* x, double* y, int32_t precision);
+void fs_matrix_destroy(fs_matrix* M);
+
+/* device memory cap used for FS_ERR_OUT_OF_MEMORY tests (0: none) */
+void fs_set_device_memory_cap(int64_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+/*
+ * oracle.h -- CPU restatement of FELSPA's Stokes solve. TEST INFRASTRUCTURE
+ * ONLY: used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg as the checker. Nothing in felspa_amd/ links or calls it.
+ *
+ * Storage follows deal.II's SparseMatrix for square matrices: the diagonal is
+ * the first entry of each row, the other columns follow in ascending order.
+ * Inputs are plain sorted CSR (diagonal in place); the oracle converts.
+ */
+#ifndef FS_ORACLE_H
+#define FS_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_csr {
+  int64_t n_rows, n_cols;
+  const int64_t* rp;
+  const int32_t* ci;
+  const double* v;
+} or_csr;
+
+/* y = M x (deal.II SparseMatrix::vmult order: diagonal first for square) */
+void or_spmv(const or_csr* M, const double* x, double* y);
+
+/* SparseILU<double> */
+typedef struct or_ilu or_ilu;
+/* initialize(): symbolic (ilu.implement.h:58-435) + numeric (:438-541).
+ * force_generic=1 uses symbolic_factorize_generic even for k=1
+ * (FORCE_SERIAL_SYMBOLIC_FACTORIZE, ilu.implement.h:21). */
+int or_ilu_create(const or_csr* A, int k, int force_generic, or_ilu** out);
+int or_ilu_refactor(or_ilu* ilu, const or_csr* A); /* use_previous_sparsity = true */
+void or_ilu_vmult(const or_ilu* ilu, const double* src, double* dst);
+void or_ilu_Tvmult(const or_ilu* ilu, const double* src, double* dst);
+/* factor in sorted CSR layout (diagonal in place); NULL arrays query sizes */
+void or_ilu_get(const or_ilu* ilu, int64_t* n, int64_t* nnz, int64_t* rp, int32_t* ci, double* v);
+void or_ilu_destroy(or_ilu* ilu);
+
+typedef struct or_opts {
+  int ilu_k_A, ilu_k_S;
+  int restart;          /* max_n_tmp_vectors - 2 */
+  double rtol;          /* tol = rtol ||rhs|| */
+  double reduce;        /* ReductionControl reduce */
+  double inner_tol;     /* InverseMatrixCG tol = reduce */
+  int apply_diagonal_scaling;
+  int method;           /* 0 FC (GMRES), 1 SCR */
+  int threads;          /* OpenMP threads for SpMV / vector ops (0: all) */
+  int max_steps;        /* 0: n */
+} or_opts;
+
+typedef struct or_stats {
+  int status;           /* 0 converged, 1 not converged */
+  int outer_iters;      /* SolverControl::last_step */
+  long long inner_iters;
+  double last_value;
+  double t_setup_s, t_krylov_s;
+} or_stats;
+
+/* StokesLinearSystem::solve (stokes_ilu.cc:442-561 / 564-710). x in: guess,
+ * out: solution (constraints are not distributed). Blocks as plain CSR. */
+int or_stokes_solve(const or_csr* A, const or_csr* Bt, const or_csr* B, const or_csr* K11,
+                    const or_csr* Mp, const or_opts* opts, const double* rhs, double* x,
+                    or_stats* stats);
+
+/* BlockSchurPreconditionerBase::vmult with fresh ILU(kA)/ILU(kS) factors
+ * and inner CG tolerance inner_tol, CG started from zero. */
+int or_block_schur_vmult(const or_csr* A, const or_csr* B, const or_csr* Mp, int kA, int kS,
+                         double inner_tol, const double* src, double* dst, long long* inner_iters);
+
+/* StokesScalingOperator::initialize (stokes_common.implement.h:13-53):
+ * writes s_u[n_u] and s_p[n_p]. */
+void or_scaling_coeffs(const or_csr* A, const or_csr* Bt, int64_t n_p, double* s_u, double* s_p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

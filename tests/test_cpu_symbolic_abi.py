@@ -1,0 +1,100 @@
+"""Host-side parts of the library (no GPU): the symbolic ILU(k) phase
+against the oracle, input validation, defaults, and that the C-ABI library
+loads and exports every symbol its headers declare."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import felspa_amd as fa
+from felspa_amd import _lib as L
+
+from conftest import ROOT, system
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3])
+@pytest.mark.parametrize("cfg,n", [("C1", 8), ("C2", 8), ("C3", 3)])
+def test_symbolic_matches_oracle(oracle, cfg, n, k):
+    A = system(cfg, n).blocks["A"]
+    P = fa.ilu_symbolic(A, k)
+    F = oracle.SparseILU(A, k).factors()
+    assert (P.indptr == F.indptr).all() and (P.indices == F.indices).all()
+
+
+def test_symbolic_accepts_diagonal_first_rows(oracle):
+    A = system("C1", 6).blocks["A"]
+    # deal.II layout: diagonal first in each row
+    rows = []
+    for r in range(A.shape[0]):
+        cols = list(A.indices[A.indptr[r]:A.indptr[r + 1]])
+        cols.remove(r)
+        rows.append([r] + cols)
+    rp = np.zeros(A.shape[0] + 1, np.int64)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    D = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=A.shape)
+    D.has_sorted_indices = False
+    P = fa.ilu_symbolic(D, 2)
+    F = oracle.SparseILU(A, 2).factors()
+    assert (P.indptr == F.indptr).all() and (P.indices == F.indices).all()
+
+
+def test_symbolic_rejects_bad_input():
+    with pytest.raises(L.FsException) as e:
+        fa.ilu_symbolic(sp.csr_matrix(np.ones((2, 3))), 0)
+    assert e.value.code == L.FS_ERR_BAD_INPUT
+    bad = sp.csr_matrix((np.ones(1), np.array([5], np.int32), np.array([0, 1, 1])), shape=(2, 2),
+                        )
+    with pytest.raises(Exception):
+        fa.ilu_symbolic(bad, 0)
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(fs_[A-Za-z_0-9]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    declared = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            declared |= _declared(h)
+    assert "fs_stokes_solve" in declared and "fs_system_create" in declared
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert set(L.EXPORTS) <= declared
+
+
+def test_default_options_follow_reference():
+    o = fa.stokes.default_opts()
+    assert o.restart == 23           # max_n_tmp_vectors = 25 (stokes_ilu.cc:538)
+    assert o.rtol == 1e-8            # tol = 1e-8 ||rhs|| (stokes_ilu.cc:533)
+    assert o.reduce == 1e-6          # StokesSolverControl reduce (stokes_common.h:343)
+    assert o.inner_cg_tol == 1e-12   # InverseMatrixCG (linear_operators.h:44-45)
+    assert o.ilu_k_A == 1 and o.ilu_k_S == 0  # set_level_of_fill defaults
+
+
+def test_create_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    S = system("C1", 4)
+    with pytest.raises(L.FsException) as e:
+        fa.StokesLinearSystem(S.blocks)
+    assert e.value.code == L.FS_ERR_NO_DEVICE
+
+
+def test_solver_control_statistics(tmp_path):
+    c = fa.StokesSolverControl()
+    c.n_gmres_iter.append(5)
+    c.gmres_error.append(1e-9)
+    c.gmres_timer.append(0.1)
+    c.log_gmres = True
+    c.write_statistics(str(tmp_path / "stats"))
+    lines = open(str(tmp_path / "stats.csv")).read().splitlines()
+    assert lines[0] == "gmres_error,gmres_timer,n_gmres_iter,"
+    assert lines[1].startswith("1e-09,0.1,5")
