@@ -1,0 +1,198 @@
+"""Stokes saddle-point solve benchmark on one MI355X.
+
+One step = one full solve, StokesLinearSystem::solve: numeric ILU(k)
+factorisation of A and M_p plus the Krylov solve to the FP64 true residual
+||b - K x|| <= 1e-8 ||b||. The rhs and x are already resident in HBM when
+the timed region starts. The headline value is the mixed-precision solve
+(FP32 SpMV / ILU / inner Krylov inside the FP64 outer FGMRES); the plain FP64
+GPU solve of the same algorithm is timed beside it.
+
+    python bench.py [--steps K] [--warmup W] [--config C2] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time-to-rtol-1e-8 per Stokes solve, mixed+learned vs plain FP64 GPU; HBM GB/s")
+WORKLOADS = {
+    "C2": "2D Rayleigh-Taylor, Q2-Q1 on 256x256 quads, interface y=0.5+0.02cos(2 pi x), "
+          "viscosity 1/100, density 1/1.1, unit gravity, free slip; ILU(1) on A, ILU(0) on M_p",
+    "C1": "2D inclusion, Q2-Q1 on 64x64 quads, viscosity contrast 1e3 disk r=0.1, simple shear, "
+          "periodic x; ILU(0)",
+    "C3": "3D porphyroclast, Q2-Q1 on 32^3 hexes, sphere r=0.2 contrast 1e3, linear simple shear; "
+          "ILU(1)",
+    "C4": "2D 128x128 seeded ellipses, simple shear; ILU(1)",
+}
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--n-cells", type=int, default=None)
+    ap.add_argument("--inner-a", type=int, default=30)
+    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--hint", type=int, default=9)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; the solve is not sharded across GPUs "
+              "(see the sharding design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import felspa_amd as fa
+    from felspa_amd import _lib as L
+
+    cfg = args.config
+    settings = fa.solver_settings(cfg)
+    kA, kS = settings["ilu_k_A"], settings["ilu_k_S"]
+    t0 = time.time()
+    S = fa.StokesSystem(fa.make_problem(cfg, n_cells=args.n_cells))
+    log("assembled %s: n_u=%d n_p=%d nnz(A)=%d in %.1fs" % (cfg, S.n_u, S.n_p, S.blocks["A"].nnz,
+                                                            time.time() - t0))
+    hint = S.block_hint(args.hint)
+    common = dict(A_lof=kA, S_lof=kS, block_hint=hint, constraints=S.constrained,
+                  restart=args.restart, inner_a_iters=args.inner_a)
+    t0 = time.time()
+    mixed = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO, **common)
+    log("mixed handle: setup (orderings + symbolic ILU + upload) %.1fs" % (time.time() - t0))
+    plain = None
+    if not args.no_plain:
+        plain = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO, **common)
+
+    dev = torch.device("cuda:0")
+    d_rhs = torch.from_numpy(S.rhs).to(dev)
+    d_x = torch.zeros(S.n, dtype=torch.float64, device=dev)
+
+    def solve(lin):
+        rc, st = lin.solve_device(d_rhs.data_ptr(), d_x.data_ptr())
+        if rc != 0:
+            raise RuntimeError("solve did not converge: %s" % st)
+        return st
+
+    def timed(lin, steps, warmup, label):
+        for i in range(warmup):
+            st = solve(lin)
+            log("%s warmup %d: %.1f ms, outer %d, inner %d, res %.2e" % (
+                label, i, st["t_total_ms"], st["outer_iters"], st["inner_iters"], st["true_rel_residual"]))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        sts = [solve(lin) for _ in range(steps)]
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        log("%s: %.1f ms/solve over %d solves (device setup %.1f + krylov %.1f ms)" % (
+            label, ms, steps, sts[-1]["t_setup_ms"], sts[-1]["t_krylov_ms"]))
+        return ms, sts[-1]
+
+    ms_mixed, st_mixed = timed(mixed, args.steps, args.warmup, "mixed")
+    x_mixed = d_x.cpu().numpy().copy()
+    out_extra = {}
+    if plain is not None:
+        ms_plain, st_plain = timed(plain, max(1, args.steps), min(args.warmup, 1), "plain-fp64")
+        x_plain = d_x.cpu().numpy().copy()
+        out_extra.update(plain_fp64_ms=round(ms_plain, 3), speedup_vs_plain_fp64=round(ms_plain / ms_mixed, 3),
+                         plain_outer_iters=st_plain["outer_iters"], plain_inner_iters=st_plain["inner_iters"])
+    # accuracy: against a tight FP64 GPU solve (rtol 1e-12) of the same system
+    ref_lin = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO, rtol=1e-12,
+                                    **{k: v for k, v in common.items()})
+    solve(ref_lin)
+    x_ref = d_x.cpu().numpy().copy()
+    del ref_lin
+
+    def errs(x):
+        nu = S.n_u
+        eu = np.linalg.norm(x[:nu] - x_ref[:nu]) / np.linalg.norm(x_ref[:nu])
+        dp = x[nu:] - x_ref[nu:]
+        tp = x_ref[nu:]
+        ep = np.linalg.norm(dp) / np.linalg.norm(tp)
+        ep0 = np.linalg.norm(dp - dp.mean()) / np.linalg.norm(tp - tp.mean())
+        return round(float(eu), 10), round(float(ep), 10), round(float(ep0), 10)
+
+    eu, ep, ep0 = errs(x_mixed)
+    out_extra.update(err_u_vs_tight=eu, err_p_vs_tight=ep, err_p_meanfree_vs_tight=ep0)
+    if plain is not None:
+        e = errs(x_plain)
+        out_extra.update(plain_err_u_vs_tight=e[0], plain_err_p_vs_tight=e[1])
+
+    # roofline: dominant kernel = the inner-PCG A-block SpMV (FP32) unless the
+    # ILU application costs more per solve; both are timed with HIP events on
+    # the solver's stream
+    kinds = {0: "A-block SpMV (k_spmv_dot, FP32)", 2: "ILU(k) apply on A (k_ilu_stage, FP32)",
+             1: "K SpMV (k_spmv, FP64)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
+    timing = {}
+    for kind in (0, 2, 1, 3):
+        ms, by = mixed.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED, reps=30)
+        timing[kind] = (ms, by)
+        log("kernel %-40s %.4f ms/launch  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
+    n_inner = st_mixed["inner_iters"]
+    dom = 0
+    ms_dom, by_dom = timing[dom]
+    achieved = by_dom / (ms_dom * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kinds[dom],
+                "bytes_per_launch": by_dom, "ms_per_launch": round(ms_dom, 5)}
+    out_extra["kernel_gbs"] = {kinds[k]: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in timing.items()}
+    out_extra["solve_gbs_algorithmic"] = round(st_mixed["bytes_algorithmic"] / (st_mixed["t_total_ms"] * 1e-3) / 1e9, 1)
+
+    # CPU baseline: the oracle's FP64 restatement of the same algorithm on the
+    # host cores, on a bounded sample (a few outer iterations), scaled to one
+    # solve by the plain FP64 GPU outer iteration count
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        threads = min(os.cpu_count() or 1, 16)
+        outer = args.cpu_outer
+        t = time.perf_counter()
+        _, cst = oracle_py.stokes_solve(S.blocks, S.rhs, kA=kA, kS=kS, restart=args.restart, rtol=1e-8,
+                                        method=2, threads=threads, max_steps=outer,
+                                        inner_a_iters=args.inner_a)
+        wall = time.perf_counter() - t
+        per_outer = cst["t_krylov_s"] / max(1, cst["outer_iters"])
+        n_outer = (out_extra.get("plain_outer_iters") or st_mixed["outer_iters"])
+        value = (cst["t_setup_s"] + per_outer * n_outer) * 1e3
+        cpu = {"value": round(value, 1), "unit": "ms", "cores": threads, "kind": "port",
+               "sample": "oracle FP64 FGMRES(%d) + inner ILU(%d)-PCG(%d) on A + CG/ILU(0) on M_p, same "
+                         "matrices: numeric ILU (%.2f s) + %d outer iterations (%.2f s, %.1f s wall), "
+                         "scaled to the %d outer iterations of the FP64 GPU solve; ILU application "
+                         "serial as in the reference, SpMV/vector ops on %d threads" % (
+                             args.restart, kA, args.inner_a, cst["t_setup_s"], cst["outer_iters"],
+                             cst["t_krylov_s"], wall, n_outer, threads)}
+        log("cpu baseline:", cpu)
+
+    line = {
+        "metric": METRIC, "value": round(ms_mixed, 3), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_mixed, 3),
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic (generated Q2-Q1 system)",
+        "config": {"workload": WORKLOADS.get(cfg, cfg), "config_id": cfg, "n_u": S.n_u, "n_p": S.n_p,
+                   "nnz_A": int(S.blocks["A"].nnz), "restart": args.restart, "inner_a_pcg": args.inner_a,
+                   "block_nodes": args.hint, "guess": "zero", "learned_guess": False},
+        "outer_iters": st_mixed["outer_iters"], "inner_iters": n_inner,
+        "true_rel_residual": st_mixed["true_rel_residual"],
+        "t_setup_ms": round(st_mixed["t_setup_ms"], 3), "t_krylov_ms": round(st_mixed["t_krylov_ms"], 3),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    line.update(out_extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

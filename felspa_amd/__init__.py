@@ -4,9 +4,10 @@ saddle-point solve (StokesLinearSystem::solve), behind a C-ABI
 from . import _lib
 from .configs import CONFIGS, make_problem, solver_settings
 from .stokes import (FC, SCR, CompareTest, NoConvergence, SparseILU, SparseMatrix,
-                     StokesLinearSystem, StokesSolverControl, ilu_symbolic)
+                     StokesLinearSystem, StokesSolverControl, block_color_ordering,
+                     ilu_symbolic)
 from .system import StokesSystem, problem
 
 __all__ = ["StokesLinearSystem", "StokesSolverControl", "SparseILU", "SparseMatrix",
            "NoConvergence", "StokesSystem", "problem", "CONFIGS", "make_problem",
-           "solver_settings", "ilu_symbolic", "FC", "SCR", "CompareTest", "_lib"]
+           "solver_settings", "ilu_symbolic", "block_color_ordering", "FC", "SCR", "CompareTest", "_lib"]

@@ -34,7 +34,7 @@ class FsOpts(C.Structure):
                 ("reduce", f64), ("inner_cg_max", i32), ("inner_cg_tol", f64),
                 ("inner_cg_tol_mixed", f64), ("inner_cg_max_mixed", i32),
                 ("apply_diagonal_scaling", i32), ("guess", i32), ("block_rows", i32),
-                ("block_hint", P(i32)), ("verbose", i32)]
+                ("block_hint", P(i32)), ("verbose", i32), ("inner_a_iters", i32)]
 
 
 class FsStats(C.Structure):
@@ -65,9 +65,9 @@ class FsSystemInfo(C.Structure):
 EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
-    "fs_stokes_set_guess_field", "fs_stokes_get_ordering", "fs_last_error", "fs_stokes_destroy",
+    "fs_stokes_set_guess_field", "fs_stokes_get_ordering", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_Tvmult", "fs_ilu_get_factors",
-    "fs_ilu_destroy", "fs_ilu_symbolic", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
+    "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
     "fs_system_create", "fs_system_destroy", "fs_system_get_info", "fs_system_csr",
     "fs_system_rhs", "fs_system_constraints", "fs_system_distribute", "fs_system_dof_maps",
@@ -96,6 +96,7 @@ def lib():
         "fs_stokes_precond_vmult": (i32, [V, P(f64), P(f64), i32, P(FsStats)]),
         "fs_stokes_set_guess_field": (i32, [V, V, P(f32), i32, P(i64), P(C.c_int8), P(i64)]),
         "fs_stokes_get_ordering": (i32, [V, i32, P(i32), P(i32)]),
+        "fs_stokes_time_kernel": (i32, [V, i32, i32, i32, P(f64), P(f64)]),
         "fs_last_error": (C.c_char_p, [V]),
         "fs_stokes_destroy": (None, [V]),
         "fs_ilu_create": (i32, [P(FsCsr), i32, i32, i32, P(i32), P(V)]),
@@ -106,6 +107,8 @@ def lib():
         "fs_ilu_destroy": (None, [V]),
         "fs_ilu_symbolic": (i32, [P(FsCsr), i32, P(P(i64)), P(P(i32))]),
         "fs_free": (None, [V]),
+        "fs_block_color_ordering": (i32, [P(FsCsr), i32, i32, P(i32), P(i32), P(i32), P(i32),
+                                          P(i32), P(i32)]),
         "fs_matrix_create": (i32, [P(FsCsr), P(V)]),
         "fs_matrix_vmult": (i32, [V, P(f64), P(f64), i32]),
         "fs_matrix_destroy": (None, [V]),

@@ -84,6 +84,23 @@ int fs_ilu_symbolic(const fs_csr* A, int32_t k, int64_t** row_ptr, int32_t** col
   FS_GUARD_END
 }
 
+int fs_block_color_ordering(const fs_csr* A, int32_t k, int32_t block_rows, const int32_t* block_hint,
+                            int32_t* perm, int32_t* blk_ptr, int32_t* n_blocks, int32_t* stage_ptr,
+                            int32_t* n_stages) {
+  FS_GUARD_BEGIN
+  HostCsr hA = csr_from_abi(A, "A", false);
+  if (hA.n_rows != hA.n_cols) throw FsError(FS_ERR_BAD_INPUT, "matrix not square");
+  if (k < 0 || k > 20) throw FsError(FS_ERR_BAD_INPUT, "level of fill out of range");
+  Schedule s = schedule_block_color(pattern_of(hA), k, block_rows > 0 ? block_rows : 64, block_hint, 256);
+  if (perm) std::copy(s.perm.begin(), s.perm.end(), perm);
+  if (blk_ptr) std::copy(s.blk_ptr.begin(), s.blk_ptr.end(), blk_ptr);
+  if (stage_ptr) std::copy(s.stage_ptr.begin(), s.stage_ptr.end(), stage_ptr);
+  if (n_blocks) *n_blocks = s.n_blocks();
+  if (n_stages) *n_stages = s.n_stages();
+  return FS_OK;
+  FS_GUARD_END
+}
+
 int fs_ilu_create(const fs_csr* A, int32_t k, int32_t ordering, int32_t block_rows, const int32_t* block_hint,
                   fs_ilu** out) {
   if (!out) { fs::set_error("fs_ilu_create: out is null"); return FS_ERR_BAD_INPUT; }

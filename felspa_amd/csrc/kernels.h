@@ -653,6 +653,107 @@ __global__ __launch_bounds__(BLOCK) void k_inner_cg(InnerCgArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Inner PCG on the velocity block A (the A^{-1} of the block preconditioner),
+// one launch per step; the scalars alpha/beta are formed inside the consuming
+// kernel from the producer's per-workgroup partial sums, summed in a fixed
+// order by every workgroup, so no separate reduction launch is needed.
+
+// sum of nb partials (every workgroup computes the same value)
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb, double* sh) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  s = block_sum(s, sh);
+  __shared__ double tot;
+  if (threadIdx.x == 0) tot = s;
+  __syncthreads();
+  return tot;
+}
+
+// q = A p ; partial[b] = sum over the workgroup's rows of p_i q_i
+// (grid-stride over rows so the partial count stays bounded)
+template <typename T, int G>
+__global__ __launch_bounds__(BLOCK) void k_spmv_dot(int64_t nrows, const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const T* __restrict__ val,
+                                                    const T* __restrict__ p, T* __restrict__ q,
+                                                    double* __restrict__ partial) {
+  __shared__ double sh[BLOCK / WAVE];
+  const int lane = threadIdx.x % G;
+  const int64_t rows_per_pass = (int64_t)gridDim.x * (BLOCK / G);
+  double contrib = 0;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows; row += rows_per_pass) {
+    const int64_t p0 = rp[row], p1 = rp[row + 1];
+    T acc = 0;
+    int64_t k = p0 + lane;
+    for (; k + G < p1; k += 2 * G) {
+      const int32_t c0 = ld_nt(ci + k), c1 = ld_nt(ci + k + G);
+      const T v0 = ld_nt(val + k), v1 = ld_nt(val + k + G);
+      acc += v0 * p[c0];
+      acc += v1 * p[c1];
+    }
+    if (k < p1) acc += ld_nt(val + k) * p[ld_nt(ci + k)];
+    acc = group_reduce<G>(acc);
+    if (lane == 0) {
+      q[row] = acc;
+      contrib += (double)acc * (double)p[row];
+    }
+  }
+  contrib = block_sum(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = contrib;
+}
+
+// alpha = rz / sum(pq partials); x += alpha p; r -= alpha q
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(int64_t n, const double* __restrict__ pq_part, int nb,
+                                                      const double* __restrict__ rz, const T* __restrict__ p,
+                                                      const T* __restrict__ q, T* __restrict__ x,
+                                                      T* __restrict__ r, int first) {
+  __shared__ double sh[BLOCK / WAVE];
+  const double pq = sum_partials(pq_part, nb, sh);
+  const double a = pq != 0.0 ? rz[0] / pq : 0.0;
+  const T alpha = (T)a;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const T pi = p[i];
+    x[i] = first ? alpha * pi : x[i] + alpha * pi;
+    r[i] -= alpha * q[i];
+  }
+}
+
+// partial[b] = sum r_i z_i
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_dot_partial_t(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
+                                                         double* __restrict__ partial) {
+  __shared__ double sh[BLOCK / WAVE];
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    s += (double)a[i] * (double)b[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// rz_new = sum(rz partials); beta = rz_new / rz (0 on the first step);
+// p = z + beta p; rz <- rz_new (workgroup 0 writes after every workgroup read)
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_pcg_direction(int64_t n, const double* __restrict__ rz_part, int nb,
+                                                         const double* __restrict__ rz_old,
+                                                         double* __restrict__ rz_new, const T* __restrict__ z,
+                                                         T* __restrict__ p, int first) {
+  __shared__ double sh[BLOCK / WAVE];
+  const double rzn = sum_partials(rz_part, nb, sh);
+  const double b = (first || rz_old[0] == 0.0) ? 0.0 : rzn / rz_old[0];
+  const T beta = (T)b;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    p[i] = first ? z[i] : z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) rz_new[0] = rzn;
+}
+
+// r = (T) src
+template <typename TS, typename T>
+__global__ __launch_bounds__(BLOCK) void k_copy_convert(int64_t n, const TS* __restrict__ src, T* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    dst[i] = (T)src[i];
+}
+
+// ---------------------------------------------------------------------------
 // Givens / least-squares update of one GMRES column (deal.II SolverGMRES
 // documented algorithm): the column is h1 + h2 (the two classical
 // Gram-Schmidt passes) over the new norm; previous rotations are applied, a new one is

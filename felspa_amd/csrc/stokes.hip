@@ -73,11 +73,13 @@ struct fs_handle {
   fs_opts o{};
   int64_t nu = 0, np = 0, n = 0, ld = 0;
   hipStream_t st = nullptr;
-  DevCsr K, Mp;
+  DevCsr K, Mp, Ablk;                                     // K, M_p and A alone, solver order
   std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp;  // caller entry -> K / Mp position
-  std::vector<double> hK, hMp;                          // host value staging
+  std::vector<int64_t> ablk_from_k;                       // A entry -> K position
+  std::vector<double> hK, hMp, hA;                        // host value staging
   IluDev FA, FS;
-  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin;
+  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
+  int64_t ldu = 0;
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
   Counters cnt;
@@ -184,10 +186,58 @@ struct fs_handle {
 
   // BlockSchurPreconditionerBase::vmult (linear_operators.implement.h:200-209):
   // dst_u = ILU_A^{-1} src_u ; t = B dst_u - src_p ; dst_p = CG(M_p, ILU_S; t)
+  // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0
+  template <typename T>
+  void inner_pcg_A(const double* src, T* x, int iters) {
+    T* base = (T*)pcg.p;
+    T *r = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
+    double* part_pq = dptr<double>(pcg_part);
+    double* part_rz = part_pq + 1024;
+    double* rz = part_pq + 2048;  // two slots
+    const int gv = 512;
+    const int gs = std::min<int>(1024, (int)std::max<int64_t>(1, (nu * Ablk.G + BLOCK - 1) / BLOCK));
+    const T* av = std::is_same<T, double>::value ? (const T*)Ablk.v64.p : (const T*)Ablk.v32.p;
+    hipLaunchKernelGGL((k_copy_convert<double, T>), dim3(gv), dim3(BLOCK), 0, st, nu, src, r);
+    FA.apply<T, T>(r, y, z, st, cnt);
+    hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
+    hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
+    cnt.launches += 3;
+    int cur = 0;
+    for (int it = 0; it < iters; ++it) {
+      launch_spmv_dot<T>(gs, av, pv, q, part_pq);
+      hipLaunchKernelGGL((k_pcg_update<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_pq, gs, rz + cur, pv, q, x, r,
+                         it == 0 ? 1 : 0);
+      cnt.launches += 2;
+      cnt.bytes += Ablk.nnz * (sizeof(T) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
+      if (it + 1 == iters) break;
+      FA.apply<T, T>(r, y, z, st, cnt);
+      hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
+      hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
+                         rz + (cur ^ 1), z, pv, 0);
+      cnt.launches += 2;
+      cnt.bytes += sizeof(T) * 5.0 * nu;
+      cur ^= 1;
+    }
+  }
+
+  template <typename T>
+  void launch_spmv_dot(int g, const T* av, const T* pv, T* q, double* part) {
+#define FS_SD(GG) \
+  case GG: hipLaunchKernelGGL((k_spmv_dot<T, GG>), dim3(g), dim3(BLOCK), 0, st, nu, Ablk.rp.as<int64_t>(), \
+                              Ablk.ci.as<int32_t>(), av, pv, q, part); break;
+    switch (Ablk.G) {
+      FS_SD(4) FS_SD(8) FS_SD(16) FS_SD(32)
+      default: FS_SD(64)
+    }
+#undef FS_SD
+  }
+
   template <typename T>
   void precond(const double* src, T* dst) {
     T* y = (T*)ytmp.p;
-    FA.apply<T, double>(src, y, dst, st, cnt);
+    const int ia = o.method == FS_METHOD_GMRES_REF ? 0 : o.inner_a_iters;
+    if (ia > 0) inner_pcg_A<T>(src, dst, ia);
+    else FA.apply<T, double>(src, y, dst, st, cnt);
     const T* kv = std::is_same<T, double>::value ? (const T*)K.v64.p : (const T*)K.v32.p;
     spmv_launch<T, T, T, double, T, 2>(K.G, np, nu, K.rp.as<int64_t>(), K.ci.as<int32_t>(), kv, dst, (T*)tp.p,
                                        src + nu, nu, 0, st);
@@ -337,6 +387,11 @@ struct fs_handle {
     if (!K.v32.p) K.v32.alloc(4 * std::max<int64_t>(1, K.nnz));
     hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(BLOCK), 0, st, K.nnz, K.v64.as<double>(),
                        K.v32.as<float>());
+    for (size_t e = 0; e < ablk_from_k.size(); ++e) hA[e] = hK[ablk_from_k[e]];
+    Ablk.v64.upload(hA.data(), 8 * Ablk.nnz);
+    if (!Ablk.v32.p) Ablk.v32.alloc(4 * std::max<int64_t>(1, Ablk.nnz));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(BLOCK), 0, st, Ablk.nnz, Ablk.v64.as<double>(),
+                       Ablk.v32.as<float>());
     Mp.v64.upload(hMp.data(), 8 * Mp.nnz);
     if (!Mp.v32.p) Mp.v32.alloc(4 * std::max<int64_t>(1, Mp.nnz));
     hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(BLOCK), 0, st, Mp.nnz, Mp.v64.as<double>(),
@@ -408,6 +463,24 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
   }
   h.K.upload_pattern(Kh);
   h.hK.assign(Kh.nnz(), 0.0);
+  // A alone (velocity rows, velocity columns) in solver order
+  HostCsr Ah;
+  Ah.n_rows = Ah.n_cols = nu;
+  Ah.rp.assign(nu + 1, 0);
+  for (int64_t r = 0; r < nu; ++r) {
+    int64_t c = 0;
+    for (int64_t q = Kh.rp[r]; q < Kh.rp[r + 1]; ++q) c += Kh.ci[q] < nu;
+    Ah.rp[r + 1] = Ah.rp[r] + c;
+  }
+  Ah.ci.resize(Ah.rp[nu]);
+  h.ablk_from_k.resize(Ah.rp[nu]);
+  for (int64_t r = 0; r < nu; ++r) {
+    int64_t o = Ah.rp[r];
+    for (int64_t q = Kh.rp[r]; q < Kh.rp[r + 1]; ++q)
+      if (Kh.ci[q] < nu) { Ah.ci[o] = Kh.ci[q]; h.ablk_from_k[o] = q; ++o; }
+  }
+  h.Ablk.upload_pattern(Ah);
+  h.hA.assign(Ah.nnz(), 0.0);
 }
 
 void fill_stats_err(fs_stats* s, int code) {
@@ -576,6 +649,10 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.tp.alloc(8 * h.ld);
   h.cg.alloc(8 * 4 * ((h.np + 63) & ~int64_t(63)));
   h.vin.alloc(8 * h.ld);
+  h.ldu = (h.nu + 63) & ~int64_t(63);
+  h.pcg.alloc(8 * 5 * h.ldu);
+  h.pcg_part.alloc(8 * 4096);
+  hip_check(hipMemset(h.pcg_part.p, 0, 8 * 4096), "memset");
   h.partial.alloc(8 * RED_BLOCKS * MAXM);
   h.sc.alloc(8 * SC_TOTAL);
   h.gbar.alloc(sizeof(GridBarrier));
@@ -754,6 +831,66 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model*, const float*
   return FS_ERR_UNSUPPORTED;
 }
 
+int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t reps, double* ms_per_launch,
+                          double* bytes_per_launch) {
+  if (!h || reps < 1) { fs::set_error("fs_stokes_time_kernel: bad argument"); return FS_ERR_BAD_INPUT; }
+  FS_GUARD_BEGIN
+  if (h->dirty) h->numeric_factorize();
+  const bool mixed = precision == FS_PREC_MIXED;
+  const int64_t nu = h->nu;
+  // deterministic nonzero inputs in the work vectors
+  const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (h->n + BLOCK - 1) / BLOCK));
+  std::vector<double> ones(h->n, 1.0);
+  h->vin.upload(ones.data(), 8 * h->n);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, h->n, h->vin.as<double>(),
+                     (float*)h->ytmp.p);
+  void* base = h->pcg.p;
+  double* part = dptr<double>(h->pcg_part);
+  auto once = [&]() {
+    h->cnt = Counters{};
+    switch (kind) {
+      case 0: {
+        const int gs = std::min<int>(1024, (int)std::max<int64_t>(1, (nu * h->Ablk.G + BLOCK - 1) / BLOCK));
+        if (mixed)
+          h->launch_spmv_dot<float>(gs, h->Ablk.v32.as<float>(), (const float*)h->ytmp.p, (float*)base, part);
+        else
+          h->launch_spmv_dot<double>(gs, h->Ablk.v64.as<double>(), h->vin.as<double>(), (double*)base, part);
+        h->cnt.bytes = h->Ablk.nnz * ((mixed ? 4.0 : 8.0) + 4.0) + 8.0 * (nu + 1) + (mixed ? 4.0 : 8.0) * 2.0 * nu;
+        break;
+      }
+      case 1:
+        h->kmult64(h->vin.as<double>(), h->r.as<double>(), nullptr);
+        break;
+      case 2:
+        if (mixed) h->FA.apply<float, float>((const float*)h->ytmp.p, (float*)base, (float*)base + h->ldu, h->st, h->cnt);
+        else h->FA.apply<double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st, h->cnt);
+        break;
+      case 3:
+        if (mixed) h->inner_cg<float>((const float*)h->ytmp.p + nu, (float*)base);
+        else h->inner_cg<double>(h->vin.as<double>() + nu, (double*)base);
+        break;
+      default:
+        throw FsError(FS_ERR_BAD_INPUT, "unknown kernel class");
+    }
+  };
+  once();  // warm-up
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  hip_check(hipEventRecord(e0, h->st), "event");
+  for (int i = 0; i < reps; ++i) once();
+  hip_check(hipEventRecord(e1, h->st), "event");
+  hip_check(hipEventSynchronize(e1), "event sync");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_launch) *ms_per_launch = ms / reps;
+  if (bytes_per_launch) *bytes_per_launch = h->cnt.bytes;
+  return FS_OK;
+  FS_GUARD_END
+}
+
 void fs_stokes_destroy(fs_handle* h) { delete h; }
 
 }  // extern "C"
@@ -781,6 +918,7 @@ void fs_opts_default(fs_opts* o) {
   o->block_rows = 0;
   o->block_hint = nullptr;
   o->verbose = 0;
+  o->inner_a_iters = 30;
 }
 
 const char* fs_last_error(const fs_handle*) { return fs::get_error(); }

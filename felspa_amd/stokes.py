@@ -90,7 +90,7 @@ class StokesLinearSystem:
                  precision=L.FS_PREC_FP64, ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=None,
                  block_rows=0, restart=23, rtol=1e-8, reduce=1e-6, inner_cg_tol=1e-12,
                  inner_cg_tol_mixed=1e-2, inner_cg_max_mixed=50, guess=L.FS_GUESS_USER,
-                 constraints=None, max_outer=0):
+                 constraints=None, max_outer=0, inner_a_iters=30):
         self._lib = L.lib()
         self.n_u = blocks["A"].shape[0]
         self.n_p = blocks["Mp"].shape[0]
@@ -102,6 +102,7 @@ class StokesLinearSystem:
         o.inner_cg_tol, o.inner_cg_tol_mixed, o.inner_cg_max_mixed = (inner_cg_tol, inner_cg_tol_mixed,
                                                                        inner_cg_max_mixed)
         o.guess, o.block_rows, o.max_outer = guess, block_rows, max_outer
+        o.inner_a_iters = inner_a_iters
         self._hint = None
         if block_hint is not None:
             self._hint = np.ascontiguousarray(block_hint, dtype=np.int32)
@@ -177,6 +178,14 @@ class StokesLinearSystem:
                                                C.byref(st))
         L.check(rc, "fs_stokes_precond_vmult")
         return dst, st.as_dict()
+
+    def time_kernel(self, kind, precision=L.FS_PREC_FP64, reps=20):
+        """(ms per launch, algorithmic bytes per launch) of one kernel class,
+        timed with HIP events on the solver's stream."""
+        ms, by = L.f64(), L.f64()
+        L.check(self._lib.fs_stokes_time_kernel(self.h, kind, precision, reps, C.byref(ms), C.byref(by)),
+                "fs_stokes_time_kernel")
+        return ms.value, by.value
 
     def ordering(self, which=0):
         n = self.n_u if which == 0 else self.n_p
@@ -295,3 +304,27 @@ class SparseMatrix:
                                           y.ctypes.data_as(L.P(L.f64)), precision),
                 "fs_matrix_vmult")
         return y
+
+
+def block_color_ordering(matrix, level_of_fill, block_rows=0, block_hint=None):
+    """The GPU ILU's block-colour schedule, computed on the host:
+    (perm[new] = old, blk_ptr, stage_ptr)."""
+    lib = L.lib()
+    keep = []
+    c = _csr_struct(matrix, keep)
+    n = matrix.shape[0]
+    perm = np.zeros(n, np.int32)
+    blk = np.zeros(n + 1, np.int32)
+    stg = np.zeros(n + 1, np.int32)
+    nb, ns = L.i32(), L.i32()
+    hint = None
+    if block_hint is not None:
+        hint_arr = np.ascontiguousarray(block_hint, np.int32)
+        keep.append(hint_arr)
+        hint = hint_arr.ctypes.data_as(L.P(L.i32))
+    L.check(lib.fs_block_color_ordering(C.byref(c), level_of_fill, block_rows, hint,
+                                        perm.ctypes.data_as(L.P(L.i32)),
+                                        blk.ctypes.data_as(L.P(L.i32)), C.byref(nb),
+                                        stg.ctypes.data_as(L.P(L.i32)), C.byref(ns)),
+            "fs_block_color_ordering")
+    return perm, blk[:nb.value + 1], stg[:ns.value + 1]

@@ -78,6 +78,9 @@ typedef struct fs_opts {
   int32_t block_rows;    /* target rows per block for FS_ORDER_BLOCK_COLOR (0: default) */
   const int32_t* block_hint; /* optional [n_u] block id per velocity row (e.g. geometric boxes) */
   int32_t verbose;
+  int32_t inner_a_iters; /* A^{-1} in the block preconditioner: 0 = one ILU(k)
+                            application (the reference's APreconditioner),
+                            >0 = that many ILU(k)-preconditioned CG steps on A */
 } fs_opts;
 
 void fs_opts_default(fs_opts* opts);
@@ -142,6 +145,15 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const 
  * M_p (which=1) blocks: perm[new] = old. */
 int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* n_stages);
 
+/* Kernel timing for roofline reports: runs `reps` launches of one kernel
+ * class on the handle's own stream between HIP events, with the current
+ * factorisation and solver-order data, and returns the mean duration of one
+ * launch (ms) and its algorithmic HBM bytes. kind: 0 = A-block SpMV of the
+ * inner PCG (q = A p with p.q), 1 = K SpMV (FP64 true residual), 2 = one
+ * ILU(k) application on A (all stage launches), 3 = one inner CG on M_p. */
+int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t reps, double* ms_per_launch,
+                          double* bytes_per_launch);
+
 const char* fs_last_error(const fs_handle* h);
 void fs_stokes_destroy(fs_handle* h);
 
@@ -170,6 +182,13 @@ void fs_ilu_destroy(fs_ilu* ilu);
  * fs_free. */
 int fs_ilu_symbolic(const fs_csr* A, int32_t level_of_fill, int64_t** row_ptr, int32_t** col_idx);
 void fs_free(void* p);
+
+/* Host-only: the block-colour ordering the GPU ILU uses for a square
+ * pattern (no device needed). perm[n] (perm[new] = old), blk_ptr[n+1] and
+ * stage_ptr[n+1] receive the schedule; *n_blocks / *n_stages its sizes. */
+int fs_block_color_ordering(const fs_csr* A, int32_t level_of_fill, int32_t block_rows,
+                            const int32_t* block_hint, int32_t* perm, int32_t* blk_ptr,
+                            int32_t* n_blocks, int32_t* stage_ptr, int32_t* n_stages);
 
 /* ---- SparseMatrix::vmult on the GPU (CSR SpMV) -------------------------- */
 typedef struct fs_matrix fs_matrix;

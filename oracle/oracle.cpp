@@ -495,6 +495,114 @@ void gmres_left(const Op& K, const Prec& P, int64_t n, double* x, const double* 
   } while (st == Control::ITERATE);
 }
 
+// Inner PCG on A (fixed number of steps from zero), ILU_A preconditioned:
+// the A^{-1} of the block preconditioner in FS_METHOD_FGMRES.
+struct InnerPcgA {
+  const DMat* A;
+  const or_ilu* P;
+  int iters;
+  mutable long long applies = 0;
+  void vmult(double* x, const double* b) const {
+    const int64_t n = A->m;
+    std::vector<double> r(b, b + n), z(n), p(n), q(n);
+    ilu_vmult(*P, r.data(), z.data());
+    ++applies;
+    p = z;
+    double rz = dot(r.data(), z.data(), n);
+    std::fill(x, x + n, 0.0);
+    for (int it = 0; it < iters; ++it) {
+      ::vmult(*A, p.data(), q.data());
+      const double pq = dot(p.data(), q.data(), n);
+      const double alpha = pq != 0.0 ? rz / pq : 0.0;
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+      }
+      if (it + 1 == iters) break;
+      ilu_vmult(*P, r.data(), z.data());
+      ++applies;
+      const double rzn = dot(r.data(), z.data(), n);
+      const double beta = rz != 0.0 ? rzn / rz : 0.0;
+      rz = rzn;
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+  }
+};
+
+// Block preconditioner with the inner PCG on A (block lower triangular form
+// of BlockSchurPreconditionerBase::vmult, linear_operators.implement.h:200-209)
+struct BlockSchurPcg {
+  const Stokes* S;
+  InnerPcgA Ainv;
+  InverseCG inverse_S;
+  mutable std::vector<double> tmp;
+  void vmult(double* dst, const double* src) const {
+    const int64_t nu = S->nu, np = S->np;
+    Ainv.vmult(dst, src);
+    tmp.resize(np);
+    residual(S->B, tmp.data(), dst, src + nu);
+    for (int64_t i = 0; i < np; ++i) tmp[i] *= -1.0;
+    std::fill(dst + nu, dst + nu + np, 0.0);
+    inverse_S.vmult(dst + nu, tmp.data());
+  }
+};
+
+// Flexible right-preconditioned GMRES(m) restarted from the true residual,
+// stopping on ||b - K x|| <= tol (the GPU FGMRES driver, FP64)
+template <class Op, class Prec>
+int fgmres_true(const Op& K, const Prec& P, int64_t n, double* x, const double* b, int m, double tol,
+                int max_steps, double& last) {
+  std::vector<std::vector<double>> V(m + 1, std::vector<double>(n)), Z(m, std::vector<double>(n));
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), h(m + 1), gam(m + 1), cs(m), sn(m);
+  int acc = 0;
+  for (;;) {
+    K.vmult(x, V[0].data());
+    for (int64_t i = 0; i < n; ++i) V[0][i] = b[i] - V[0][i];
+    const double beta = norm(V[0].data(), n);
+    last = beta;
+    if (beta <= tol) return acc;
+    if (acc >= max_steps) return acc;
+    for (int64_t i = 0; i < n; ++i) V[0][i] /= beta;
+    gam.assign(m + 1, 0.0);
+    gam[0] = beta;
+    int dim = 0;
+    for (int j = 0; j < m && acc < max_steps; ++j) {
+      ++acc;
+      P.vmult(Z[j].data(), V[j].data());
+      K.vmult(Z[j].data(), V[j + 1].data());
+      double* w = V[j + 1].data();
+      std::vector<double> h1(j + 1, 0.0);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i <= j; ++i) {
+          const double c = dot(w, V[i].data(), n);
+          h1[i] += c;
+          const double* vi = V[i].data();
+#pragma omp parallel for schedule(static)
+          for (int64_t t = 0; t < n; ++t) w[t] -= c * vi[t];
+        }
+      for (int i = 0; i <= j; ++i) h[i] = h1[i];
+      const double s = norm(w, n);
+      h[j + 1] = s;
+      if (s != 0)
+        for (int64_t t = 0; t < n; ++t) w[t] /= s;
+      givens(h, gam, cs, sn, j);
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h[i];
+      dim = j + 1;
+      if (std::fabs(gam[dim]) <= tol) break;
+    }
+    std::vector<double> y(dim);
+    for (int i = dim - 1; i >= 0; --i) {
+      double s = gam[i];
+      for (int k = i + 1; k < dim; ++k) s -= H[(size_t)i * m + k] * y[k];
+      y[i] = s / H[(size_t)i * m + i];
+    }
+    for (int i = 0; i < dim; ++i)
+      for (int64_t t = 0; t < n; ++t) x[t] += y[i] * Z[i][t];
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -629,7 +737,16 @@ int or_stokes_solve(const or_csr* Ac, const or_csr* Btc, const or_csr* Bc, const
   ctrl.max_steps = o->max_steps > 0 ? o->max_steps : (int)n;
   bool ok = true;
   long long inner = 0;
-  if (o->method == 0) {
+  if (o->method == 2) {
+    BlockSchurPcg P{&S, InnerPcgA{&S.A, &IA, o->inner_a_iters > 0 ? o->inner_a_iters : 30},
+                    InverseCG{&S.Mp, &IS, o->inner_tol, o->inner_tol}, {}};
+    double last = 0;
+    const int it = fgmres_true(S, P, n, x, rhs.data(), o->restart, o->rtol * bnorm, ctrl.max_steps, last);
+    ctrl.last_step = it;
+    ctrl.last_value = last;
+    ok = P.inverse_S.ok && last <= o->rtol * bnorm;
+    inner = P.inverse_S.iters + P.Ainv.applies;
+  } else if (o->method == 0) {
     BlockSchur P{&S, &IA, InverseCG{&S.Mp, &IS, o->inner_tol, o->inner_tol}, {}};
     gmres_left(S, P, n, x, rhs.data(), o->restart, ctrl);
     ok = P.inverse_S.ok && ctrl.last_value <= std::max(ctrl.tol, ctrl.reduced_tol);

@@ -44,9 +44,10 @@ typedef struct or_opts {
   double reduce;        /* ReductionControl reduce */
   double inner_tol;     /* InverseMatrixCG tol = reduce */
   int apply_diagonal_scaling;
-  int method;           /* 0 FC (GMRES), 1 SCR */
+  int method;           /* 0 FC (GMRES), 1 SCR, 2 FGMRES + inner PCG on A (GPU method) */
   int threads;          /* OpenMP threads for SpMV / vector ops (0: all) */
   int max_steps;        /* 0: n */
+  int inner_a_iters;    /* method 2: PCG steps for A^{-1} */
 } or_opts;
 
 typedef struct or_stats {

@@ -29,7 +29,7 @@ class OrOpts(C.Structure):
     _fields_ = [("ilu_k_A", C.c_int), ("ilu_k_S", C.c_int), ("restart", C.c_int),
                 ("rtol", C.c_double), ("reduce", C.c_double), ("inner_tol", C.c_double),
                 ("apply_diagonal_scaling", C.c_int), ("method", C.c_int),
-                ("threads", C.c_int), ("max_steps", C.c_int)]
+                ("threads", C.c_int), ("max_steps", C.c_int), ("inner_a_iters", C.c_int)]
 
 
 class OrStats(C.Structure):
@@ -133,14 +133,15 @@ class SparseILU:
 
 
 def stokes_solve(blocks, rhs, x0=None, kA=1, kS=0, restart=23, rtol=1e-8, reduce=1e-6,
-                 inner_tol=1e-12, scaling=False, method=0, threads=0, max_steps=0):
+                 inner_tol=1e-12, scaling=False, method=0, threads=0, max_steps=0, inner_a_iters=30):
     """StokesLinearSystem::solve on the CPU. blocks = dict of scipy CSR
     A, Bt, B, K11, Mp. Returns (x, stats dict)."""
     cs = [_Csr(blocks[k]) for k in ("A", "Bt", "B", "K11", "Mp")]
     n = blocks["A"].shape[0] + blocks["Mp"].shape[0]
     rhs = np.ascontiguousarray(rhs, dtype=np.float64)
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
-    o = OrOpts(kA, kS, restart, rtol, reduce, inner_tol, int(scaling), method, threads, max_steps)
+    o = OrOpts(kA, kS, restart, rtol, reduce, inner_tol, int(scaling), method, threads, max_steps,
+               inner_a_iters)
     st = OrStats()
     lib().or_stokes_solve(*[C.byref(c.s) for c in cs], C.byref(o), _ptr(rhs, C.c_double),
                           _ptr(x, C.c_double), C.byref(st))
