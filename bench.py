@@ -44,9 +44,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--n-cells", type=int, default=None)
-    ap.add_argument("--inner-a", type=int, default=30)
+    ap.add_argument("--inner-a", type=int, default=60)
     ap.add_argument("--restart", type=int, default=30)
-    ap.add_argument("--hint", type=int, default=9)
+    ap.add_argument("--hint", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")

@@ -186,6 +186,7 @@ static void build_transposed(fs_ilu* h) {
   if (!h->FT.v32.p) h->FT.v32.alloc(4 * h->FT.nnz);
   hipLaunchKernelGGL((k_convert<double, float>), dim3(grid1(h->FT.nnz)), dim3(BLOCK), 0, h->st, h->FT.nnz,
                      h->FT.v64.as<double>(), h->FT.v32.as<float>());
+  h->FT.gather_inblock(h->st);
   h->ft_ready = true;
 }
 

@@ -20,6 +20,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Orders LDS accesses of one wave for the compiler only: the LDS executes a
+// wave's operations in issue order, so no wait is needed between a lane's
+// write and another lane's later read.
+__device__ __forceinline__ void wave_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <int G, typename T>
 __device__ __forceinline__ T group_reduce(T v) {
 #pragma unroll
@@ -32,7 +40,7 @@ __device__ __forceinline__ T ld_nt(const T* p) {
   return __builtin_nontemporal_load(p);
 }
 
-// block-wide sum of a double into lane 0 of wave 0 (BLOCK threads)
+// block-wide sum of a double into thread 0 (blockDim.x <= BLOCK threads)
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   v = group_reduce<WAVE>(v);
   const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
@@ -87,6 +95,123 @@ __device__ __forceinline__ bool grid_sync(GridBarrier* gb, unsigned int& gen, ui
 }
 
 // ---------------------------------------------------------------------------
+// CSR row dot product with G lanes per row. Each lane keeps 4 column
+// indices, 4 values and 4 gathered x values in flight: loads go to clamped
+// in-range positions and out-of-range terms are masked afterwards, so the
+// compiler emits no per-element branch (a branch per gather serialises the
+// gathers on their memory latency).
+template <typename TV, typename TX, typename TACC, int G, bool LIMIT>
+__device__ __forceinline__ TACC row_dot(const int32_t* __restrict__ ci, const TV* __restrict__ val,
+                                        const TX* __restrict__ x, int64_t p0, int64_t p1, int lane,
+                                        int64_t col_limit, int64_t x_off) {
+  TACC acc = 0;
+  const int64_t last = p1 - 1;
+  for (int64_t k = p0 + lane; k < p1; k += 4 * G) {
+    int32_t c[4];
+    TV v[4];
+    bool in[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t kk = k + u * G;
+      in[u] = kk < p1;
+      const int64_t kc = in[u] ? kk : last;
+      c[u] = ld_nt(ci + kc);
+      v[u] = ld_nt(val + kc);
+    }
+    TX xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool ok = !LIMIT || c[u] < col_limit;
+      in[u] = in[u] && ok;
+      xv[u] = x[(ok ? (int64_t)c[u] : x_off) - x_off];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += in[u] ? (TACC)v[u] * (TACC)xv[u] : (TACC)0;
+  }
+  return group_reduce<G>(acc);
+}
+
+// Padded CSR ("CSR4"): every row starts at a multiple of 4 entries and is
+// padded to one (padding: value 0, column = a valid column of the row), so a
+// lane streams 4 consecutive entries with one 16-byte column load and one
+// 16/32-byte value load, and the G lanes of a row read one contiguous span.
+template <typename TV>
+struct Vec4;
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <>
+struct Vec4<float> {
+  using type = f32x4;
+};
+template <>
+struct Vec4<double> {
+  using type = f64x4;
+};
+
+template <typename TV, typename TX, typename TACC, int G, bool LIMIT>
+__device__ __forceinline__ TACC row_dot4(const i32x4* __restrict__ ci4, const typename Vec4<TV>::type* __restrict__ val4,
+                                         const TX* __restrict__ x, int64_t q0, int64_t q1, int lane,
+                                         int64_t col_limit) {
+  TACC acc = 0;
+  for (int64_t q = q0 + lane; q < q1; q += G) {
+    const i32x4 c = __builtin_nontemporal_load(ci4 + q);
+    const typename Vec4<TV>::type v = __builtin_nontemporal_load(val4 + q);
+    if (LIMIT) {  // clamped gathers, masked values: no branch per gather
+      const bool m0 = c.x < col_limit, m1 = c.y < col_limit, m2 = c.z < col_limit, m3 = c.w < col_limit;
+      const TX x0 = x[m0 ? c.x : 0], x1 = x[m1 ? c.y : 0], x2 = x[m2 ? c.z : 0], x3 = x[m3 ? c.w : 0];
+      acc += (TACC)(m0 ? v.x : (TV)0) * (TACC)x0 + (TACC)(m1 ? v.y : (TV)0) * (TACC)x1 +
+             (TACC)(m2 ? v.z : (TV)0) * (TACC)x2 + (TACC)(m3 ? v.w : (TV)0) * (TACC)x3;
+    } else {
+      const TX x0 = x[c.x], x1 = x[c.y], x2 = x[c.z], x3 = x[c.w];
+      acc += (TACC)v.x * (TACC)x0 + (TACC)v.y * (TACC)x1 + (TACC)v.z * (TACC)x2 + (TACC)v.w * (TACC)x3;
+    }
+  }
+  return group_reduce<G>(acc);
+}
+
+// y = M x / b - M x / M x - b on a CSR4 matrix (rp4 counts groups of 4)
+template <typename TV, typename TX, typename TY, typename TB, typename TACC, int G, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off, const int64_t* __restrict__ rp4,
+                                                 const i32x4* __restrict__ ci4,
+                                                 const typename Vec4<TV>::type* __restrict__ val4,
+                                                 const TX* __restrict__ x, TY* __restrict__ y, const TB* __restrict__ b,
+                                                 int64_t col_limit) {
+  const int64_t gid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t row = gid / G;
+  const int lane = (int)(gid % G);
+  if (row >= nrows) return;
+  const int64_t r = row + row_off;
+  const TACC acc = row_dot4<TV, TX, TACC, G, MODE == 2>(ci4, val4, x, rp4[r], rp4[r + 1], lane, col_limit);
+  if (lane == 0) {
+    if (MODE == 0) y[row] = (TY)acc;
+    else if (MODE == 1) y[row] = (TY)((TACC)b[row] - acc);
+    else y[row] = (TY)(acc - (TACC)b[row]);
+  }
+}
+
+// q = A p on a CSR4 matrix; partial[b] = sum over the workgroup's rows of p_i q_i
+template <typename T, int G>
+__global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_t* __restrict__ rp4,
+                                                     const i32x4* __restrict__ ci4,
+                                                     const typename Vec4<T>::type* __restrict__ val4,
+                                                     const T* __restrict__ p, T* __restrict__ q,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[BLOCK / WAVE];
+  const int lane = threadIdx.x % G;
+  const int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G;
+  double contrib = 0;
+  if (row < nrows) {
+    const T acc = row_dot4<T, T, T, G, false>(ci4, val4, p, rp4[row], rp4[row + 1], lane, 0);
+    if (lane == 0) {
+      q[row] = acc;
+      contrib = (double)acc * (double)p[row];
+    }
+  }
+  contrib = block_sum(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = contrib;
+}
+
 // CSR SpMV: G lanes per row, BLOCK/G rows per workgroup.
 //   MODE 0: y = M x
 //   MODE 1: y = b - M x
@@ -103,25 +228,40 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int64_t nrows, int64_t row_off,
   const int lane = (int)(gid % G);
   if (row >= nrows) return;
   const int64_t r = row + row_off;
-  const int64_t p0 = rp[r], p1 = rp[r + 1];
-  TACC acc = 0;
-  int64_t p = p0 + lane;
-  for (; p + G < p1; p += 2 * G) {
-    const int32_t c0 = ld_nt(ci + p), c1 = ld_nt(ci + p + G);
-    const TV v0 = ld_nt(val + p), v1 = ld_nt(val + p + G);
-    if (MODE != 2 || c0 < col_limit) acc += (TACC)v0 * (TACC)x[c0 - x_off];
-    if (MODE != 2 || c1 < col_limit) acc += (TACC)v1 * (TACC)x[c1 - x_off];
-  }
-  if (p < p1) {
-    const int32_t c0 = ld_nt(ci + p);
-    const TV v0 = ld_nt(val + p);
-    if (MODE != 2 || c0 < col_limit) acc += (TACC)v0 * (TACC)x[c0 - x_off];
-  }
-  acc = group_reduce<G>(acc);
+  const TACC acc = row_dot<TV, TX, TACC, G, MODE == 2>(ci, val, x, rp[r], rp[r + 1], lane, col_limit, x_off);
   if (lane == 0) {
     if (MODE == 0) y[row] = (TY)acc;
     else if (MODE == 1) y[row] = (TY)((TACC)b[row] - acc);
     else y[row] = (TY)(acc - (TACC)b[row]);
+  }
+}
+
+// Deterministic grid reduction: every workgroup stores its partial; the last
+// workgroup to finish (atomic ticket) sums all partials in index order and
+// writes the scalar, then resets the ticket for the next launch.
+__device__ __forceinline__ void grid_reduce_last(double s, double* __restrict__ partial,
+                                                 unsigned int* __restrict__ ticket, double* __restrict__ out,
+                                                 double* sh) {
+  s = block_sum(s, sh);
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double t = 0;
+    // fixed order: thread i sums partials i, i+BLOCK, ..., then a fixed tree
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK)
+      t += __hip_atomic_load(partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+      out[0] = t;
+      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -142,7 +282,7 @@ __global__ __launch_bounds__(BLOCK) void k_dot_partial(int64_t n, const T* __res
 // out[0] = sum(partial[0..nb)); mode 1: sqrt
 static __global__ void k_reduce_final(int nb, const double* __restrict__ partial, double* __restrict__ out,
                                int mode) {
-  __shared__ double sh[BLOCK / WAVE];
+  __shared__ double sh[1024 / WAVE];  // up to 1024 threads
   double s = 0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
   s = block_sum(s, sh);
@@ -259,6 +399,17 @@ __global__ __launch_bounds__(BLOCK) void k_permute(int64_t n, const int32_t* __r
   }
 }
 
+// d64[i] = src[pos[i]], d32[i] = (float) src[pos[i]]
+static __global__ __launch_bounds__(BLOCK) void k_gather_vals(int64_t n, const int64_t* __restrict__ pos,
+                                                              const double* __restrict__ src,
+                                                              double* __restrict__ d64, float* __restrict__ d32) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const double v = src[pos[i]];
+    d64[i] = v;
+    d32[i] = (float)v;
+  }
+}
+
 // dst[map[e]] = src[e]
 static __global__ __launch_bounds__(BLOCK) void k_scatter_vals(int64_t n, const int64_t* __restrict__ map,
                                                         const double* __restrict__ src,
@@ -275,8 +426,10 @@ static __global__ __launch_bounds__(BLOCK) void k_scatter_vals(int64_t n, const 
 //   o_diag             diagonal
 //   (o_diag, o_uext)   U entries inside the block
 //   [o_uext, len)      U entries in later blocks
-// lpre / upre: start of the row's in-block L / U entries within the block's
-// compacted list (prefix over the block's rows).
+// The in-block entries are also kept transposed (column order) so the
+// sequential part of a block solve pushes each finished value to the rows
+// that depend on it: lcp[c]..lcp[c+1] are the in-block L entries of column c
+// (their local rows in lrow, values in lv), ucp / urow / uv likewise for U.
 template <typename T>
 struct IluView {
   const int64_t* rp;
@@ -285,20 +438,23 @@ struct IluView {
   const int32_t* o_lint;
   const int32_t* o_diag;
   const int32_t* o_uext;
-  const int32_t* lpre;
-  const int32_t* upre;
   const int32_t* blk_ptr;
+  const int64_t* lcp;
+  const uint16_t* lrow;
+  const T* lv;
+  const int64_t* ucp;
+  const uint16_t* urow;
+  const T* uv;
 };
 
 // LDS layout of one wave's block workspace
 template <typename T>
 struct BlockLds {
-  T* s;        // [max_rows]   running values
-  T* d;        // [max_rows]   diagonal (backward)
-  int32_t* st; // [max_rows]   start in compacted list
-  int32_t* ct; // [max_rows]   count
-  T* ev;       // [cap]        in-block entry values
-  uint16_t* ec;// [cap]        in-block entry local columns
+  T* s;         // [max_rows]   running values
+  T* d;         // [max_rows]   diagonal
+  T* ev;        // [cap]        in-block entry values (column order)
+  int32_t* st;  // [max_rows+1] start of each column's entries
+  uint16_t* ec; // [cap]        in-block entry local rows
 };
 
 template <typename T>
@@ -308,133 +464,150 @@ __device__ __forceinline__ BlockLds<T> carve_lds(char* base, int max_rows, int c
   L.s = (T*)p; p += sizeof(T) * max_rows;
   L.d = (T*)p; p += sizeof(T) * max_rows;
   L.ev = (T*)p; p += sizeof(T) * cap;
-  L.st = (int32_t*)p; p += 4 * max_rows;
-  L.ct = (int32_t*)p; p += 4 * max_rows;
+  L.st = (int32_t*)p; p += 4 * (max_rows + 1);
   L.ec = (uint16_t*)p;
   return L;
 }
 
 __host__ __device__ inline size_t lds_bytes_per_wave(int max_rows, int cap, int tsize) {
-  size_t b = (size_t)tsize * (2 * max_rows + cap) + 8 * (size_t)max_rows + 2 * (size_t)cap;
+  size_t b = (size_t)tsize * (2 * max_rows + cap) + 4 * (size_t)(max_rows + 1) + 2 * (size_t)cap;
   return (b + 15) & ~(size_t)15;
 }
 
-// Forward solve of one block: y = L^{-1} b restricted to the block's rows,
-// L unit lower (UNIT=true) or with its diagonal (transposed factors).
-// One wave; G lanes per row for the part that reads earlier blocks.
-template <typename T, typename TB, typename TY, int G, bool UNIT>
+// Sequential part of a block solve. Rows are finished in order (forward:
+// 0..nr-1, backward: nr-1..0); a finished value v_i = s_i (/ d_i) is pushed
+// to its dependent rows, s_r -= e_ri v_i, one entry per lane. The next row's
+// column list is fetched while the current row is pushed, so the chain per
+// row is one LDS read, one FMA and one LDS write. EV/EC are LDS pointers, or
+// global memory for blocks whose entries did not fit.
+template <typename T, bool UNIT, bool FWD, typename PV, typename PC>
+__device__ __forceinline__ void push_phase(BlockLds<T>& L, PV ev, PC ec, int nr, int lane) {
+  if (nr <= 0) return;
+  const bool divide = FWD ? !UNIT : true;
+  int i = FWD ? 0 : nr - 1;
+  const int step = FWD ? 1 : -1;
+  int k0 = L.st[i], k1 = L.st[i + 1];
+  int k = k0 + lane;
+  bool act = k < k1;
+  int kc = act ? k : 0;
+  int row = (int)ec[kc];
+  T val = (T)ev[kc];
+  for (int it = 0; it < nr; ++it, i += step) {
+    const int cur_k0 = k0, cur_k1 = k1;
+    const bool cur_act = act;
+    const int cur_row = row;
+    const T cur_val = val;
+    // prefetch the next row's column list
+    if (it + 1 < nr) {
+      const int ni = i + step;
+      k0 = L.st[ni];
+      k1 = L.st[ni + 1];
+      k = k0 + lane;
+      act = k < k1;
+      kc = act ? k : 0;
+      row = (int)ec[kc];
+      val = (T)ev[kc];
+    }
+    T vi = L.s[i];
+    if (divide) vi /= L.d[i];
+    const T sr = cur_act ? L.s[cur_row] : (T)0;
+    if (divide && lane == 0) L.s[i] = vi;
+    if (cur_act) L.s[cur_row] = sr - cur_val * vi;
+    // rows with more than 64 dependents (3D blocks)
+    for (int kk = cur_k0 + WAVE + lane; kk < cur_k1; kk += WAVE) {
+      const int rr = ec[kk];
+      L.s[rr] -= (T)ev[kk] * vi;
+    }
+    wave_order();
+  }
+}
+
+// Forward solve of one block: y = L^{-1} b on the block's rows; L unit lower
+// (UNIT=true) or with its diagonal (transposed factors). NT threads of the
+// caller cooperate (a whole workgroup, or one wave): the part that reads
+// earlier blocks runs over all rows at once, G threads per row; the in-block
+// part then runs on the first wave (push_phase), the others wait.
+template <typename T, typename TB, typename TY, int G, bool UNIT, int NT>
 __device__ void ilu_fwd_block(const IluView<T>& F, int blk, const TB* __restrict__ b, TY* __restrict__ y,
-                              BlockLds<T> L, int cap) {
-  const int lane = threadIdx.x % WAVE;
+                              BlockLds<T> L, int cap, int tid) {
   const int r0 = F.blk_ptr[blk], r1 = F.blk_ptr[blk + 1];
   const int nr = r1 - r0;
-  const int grp = lane / G, gl = lane % G;
-  constexpr int NG = WAVE / G;
-  // phase A: contributions of earlier blocks + stage in-block entries
+  const int grp = tid / G, gl = tid % G;
+  constexpr int NG = NT / G;
+  const int64_t e0 = F.lcp[r0], ne = F.lcp[r1] - e0;
+  const bool in_lds = ne <= cap;
   for (int i = grp; i < nr; i += NG) {
     const int r = r0 + i;
     const int64_t base = F.rp[r];
-    const int olint = F.o_lint[r], odiag = F.o_diag[r];
-    T acc = 0;
-    for (int p = gl; p < olint; p += G) acc += ld_nt(F.v + base + p) * (T)y[F.ci[base + p]];
-    acc = group_reduce<G>(acc);
-    const int st = F.lpre[r], cnt = odiag - olint;
+    const int olint = F.o_lint[r];
+    const T acc = row_dot<T, TY, T, G, false>(F.ci, F.v, y, base, base + olint, gl, 0, 0);
     if (gl == 0) {
       L.s[i] = (T)b[r] - acc;
-      L.st[i] = st;
-      L.ct[i] = cnt;
-      if (!UNIT) L.d[i] = F.v[base + odiag];
+      if (!UNIT) L.d[i] = F.v[base + F.o_diag[r]];
     }
-    if (st + cnt <= cap)
-      for (int t = gl; t < cnt; t += G) {
-        L.ev[st + t] = F.v[base + olint + t];
-        L.ec[st + t] = (uint16_t)(F.ci[base + olint + t] - r0);
-      }
   }
-  wave_sync();
-  // phase B: rows in order, dependencies inside the block from LDS
-  for (int i = 0; i < nr; ++i) {
-    const int st = L.st[i], cnt = L.ct[i];
-    T acc = 0;
-    if (st + cnt <= cap) {
-      for (int t = lane; t < cnt; t += WAVE) acc += L.ev[st + t] * L.s[L.ec[st + t]];
-    } else {  // entries did not fit in LDS: read them from memory
-      const int r = r0 + i;
-      const int64_t base = F.rp[r] + F.o_lint[r];
-      for (int t = lane; t < cnt; t += WAVE) acc += F.v[base + t] * L.s[F.ci[base + t] - r0];
+  for (int i = tid; i <= nr; i += NT) L.st[i] = (int32_t)(F.lcp[r0 + i] - e0);
+  if (in_lds)
+    for (int64_t k = tid; k < ne; k += NT) {
+      L.ev[k] = F.lv[e0 + k];
+      L.ec[k] = F.lrow[e0 + k];
     }
-    if (cnt > 0) acc = group_reduce<WAVE>(acc);
-    if (lane == 0) {
-      T yi = L.s[i] - acc;
-      if (!UNIT) yi /= L.d[i];
-      L.s[i] = yi;
-    }
-    wave_sync();
+  if (NT > WAVE) __syncthreads(); else wave_sync();
+  if (tid < WAVE) {
+    if (in_lds) push_phase<T, UNIT, true>(L, L.ev, L.ec, nr, tid);
+    else push_phase<T, UNIT, true>(L, F.lv + e0, F.lrow + e0, nr, tid);
   }
-  for (int i = lane; i < nr; i += WAVE) y[r0 + i] = (TY)L.s[i];
+  if (NT > WAVE) __syncthreads(); else wave_sync();
+  for (int i = tid; i < nr; i += NT) y[r0 + i] = (TY)L.s[i];
 }
 
-// Backward solve of one block: x = U^{-1} y, U with diagonal (UNIT=false) or
-// unit upper (transposed factors).
-template <typename T, typename TB, typename TY, int G, bool UNIT>
+// Backward solve of one block: x = U^{-1} y, U with its diagonal (UNIT=false)
+// or unit upper (transposed factors); same organisation as the forward solve.
+template <typename T, typename TB, typename TY, int G, bool UNIT, int NT>
 __device__ void ilu_bwd_block(const IluView<T>& F, int blk, const TB* __restrict__ yin, TY* __restrict__ x,
-                              BlockLds<T> L, int cap) {
-  const int lane = threadIdx.x % WAVE;
+                              BlockLds<T> L, int cap, int tid) {
   const int r0 = F.blk_ptr[blk], r1 = F.blk_ptr[blk + 1];
   const int nr = r1 - r0;
-  const int grp = lane / G, gl = lane % G;
-  constexpr int NG = WAVE / G;
+  const int grp = tid / G, gl = tid % G;
+  constexpr int NG = NT / G;
+  const int64_t e0 = F.ucp[r0], ne = F.ucp[r1] - e0;
+  const bool in_lds = ne <= cap;
   for (int i = grp; i < nr; i += NG) {
     const int r = r0 + i;
     const int64_t base = F.rp[r];
     const int len = (int)(F.rp[r + 1] - base);
-    const int odiag = F.o_diag[r], ouext = F.o_uext[r];
-    T acc = 0;
-    for (int p = ouext + gl; p < len; p += G) acc += ld_nt(F.v + base + p) * (T)x[F.ci[base + p]];
-    acc = group_reduce<G>(acc);
-    const int st = F.upre[r], cnt = ouext - odiag - 1;
+    const int ouext = F.o_uext[r];
+    const T acc = row_dot<T, TY, T, G, false>(F.ci, F.v, x, base + ouext, base + len, gl, 0, 0);
     if (gl == 0) {
       L.s[i] = (T)yin[r] - acc;
-      L.st[i] = st;
-      L.ct[i] = cnt;
-      L.d[i] = UNIT ? (T)1 : F.v[base + odiag];
+      L.d[i] = UNIT ? (T)1 : F.v[base + F.o_diag[r]];
     }
-    if (st + cnt <= cap)
-      for (int t = gl; t < cnt; t += G) {
-        L.ev[st + t] = F.v[base + odiag + 1 + t];
-        L.ec[st + t] = (uint16_t)(F.ci[base + odiag + 1 + t] - r0);
-      }
   }
-  wave_sync();
-  for (int i = nr - 1; i >= 0; --i) {
-    const int st = L.st[i], cnt = L.ct[i];
-    T acc = 0;
-    if (st + cnt <= cap) {
-      for (int t = lane; t < cnt; t += WAVE) acc += L.ev[st + t] * L.s[L.ec[st + t]];
-    } else {
-      const int r = r0 + i;
-      const int64_t base = F.rp[r] + F.o_diag[r] + 1;
-      for (int t = lane; t < cnt; t += WAVE) acc += F.v[base + t] * L.s[F.ci[base + t] - r0];
+  for (int i = tid; i <= nr; i += NT) L.st[i] = (int32_t)(F.ucp[r0 + i] - e0);
+  if (in_lds)
+    for (int64_t k = tid; k < ne; k += NT) {
+      L.ev[k] = F.uv[e0 + k];
+      L.ec[k] = F.urow[e0 + k];
     }
-    if (cnt > 0) acc = group_reduce<WAVE>(acc);
-    if (lane == 0) L.s[i] = (L.s[i] - acc) / L.d[i];
-    wave_sync();
+  if (NT > WAVE) __syncthreads(); else wave_sync();
+  if (tid < WAVE) {
+    if (in_lds) push_phase<T, UNIT, false>(L, L.ev, L.ec, nr, tid);
+    else push_phase<T, UNIT, false>(L, F.uv + e0, F.urow + e0, nr, tid);
   }
-  for (int i = lane; i < nr; i += WAVE) x[r0 + i] = (TY)L.s[i];
+  if (NT > WAVE) __syncthreads(); else wave_sync();
+  for (int i = tid; i < nr; i += NT) x[r0 + i] = (TY)L.s[i];
 }
 
-// one stage: each wave takes blocks blk0 + wave_id (+ stride)
+// one stage: each workgroup solves blocks blk0 + blockIdx.x (+ stride)
 template <typename T, typename TB, typename TY, int G, bool UNIT, bool FWD>
 __global__ __launch_bounds__(BLOCK) void k_ilu_stage(IluView<T> F, int blk0, int blk1, const TB* __restrict__ b,
                                                      TY* __restrict__ y, int max_rows, int cap) {
   extern __shared__ char smem[];
-  const int wv = threadIdx.x / WAVE;
-  const size_t per_wave = lds_bytes_per_wave(max_rows, cap, sizeof(T));
-  BlockLds<T> L = carve_lds<T>(smem + per_wave * wv, max_rows, cap);
-  const int nw = gridDim.x * (BLOCK / WAVE);
-  for (int blk = blk0 + blockIdx.x * (BLOCK / WAVE) + wv; blk < blk1; blk += nw) {
-    if (FWD) ilu_fwd_block<T, TB, TY, G, UNIT>(F, blk, b, y, L, cap);
-    else ilu_bwd_block<T, TB, TY, G, UNIT>(F, blk, b, y, L, cap);
+  BlockLds<T> L = carve_lds<T>(smem, max_rows, cap);
+  for (int blk = blk0 + blockIdx.x; blk < blk1; blk += gridDim.x) {
+    if (FWD) ilu_fwd_block<T, TB, TY, G, UNIT, BLOCK>(F, blk, b, y, L, cap, threadIdx.x);
+    else ilu_bwd_block<T, TB, TY, G, UNIT, BLOCK>(F, blk, b, y, L, cap, threadIdx.x);
+    __syncthreads();
   }
 }
 
@@ -542,13 +715,10 @@ __global__ __launch_bounds__(BLOCK) void k_inner_cg(InnerCgArgs<T> a) {
   extern __shared__ char smem[];
   __shared__ double shr[BLOCK / WAVE];
   __shared__ double tot;
-  const int wv = threadIdx.x / WAVE;
-  const size_t per_wave = lds_bytes_per_wave(a.max_rows, a.cap, sizeof(T));
-  BlockLds<T> L = carve_lds<T>(smem + per_wave * wv, a.max_rows, a.cap);
+  BlockLds<T> L = carve_lds<T>(smem, a.max_rows, a.cap);
   unsigned int gen = __hip_atomic_load(&a.gb->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int nwg = gridDim.x;
   const int64_t tid = (int64_t)blockIdx.x * BLOCK + threadIdx.x, nthr = (int64_t)nwg * BLOCK;
-  const int gw = blockIdx.x * (BLOCK / WAVE) + wv, nw = nwg * (BLOCK / WAVE);
   int red = 0;
   // reduce a per-thread value over the grid; all workgroups get the sum
   auto allreduce = [&](double v, bool& ok) -> double {
@@ -567,13 +737,17 @@ __global__ __launch_bounds__(BLOCK) void k_inner_cg(InnerCgArgs<T> a) {
   };
   auto precond = [&](const T* src, T* dst, bool& ok) {
     for (int s = 0; s < a.n_stages && ok; ++s) {
-      for (int blk = a.stage_ptr[s] + gw; blk < a.stage_ptr[s + 1]; blk += nw)
-        ilu_fwd_block<T, T, T, GF, true>(a.F, blk, src, dst, L, a.cap);
+      for (int blk = a.stage_ptr[s] + blockIdx.x; blk < a.stage_ptr[s + 1]; blk += nwg) {
+        ilu_fwd_block<T, T, T, GF, true, BLOCK>(a.F, blk, src, dst, L, a.cap, threadIdx.x);
+        __syncthreads();
+      }
       ok = grid_sync(a.gb, gen, a.timeout);
     }
     for (int s = a.n_stages - 1; s >= 0 && ok; --s) {
-      for (int blk = a.stage_ptr[s] + gw; blk < a.stage_ptr[s + 1]; blk += nw)
-        ilu_bwd_block<T, T, T, GF, false>(a.F, blk, dst, dst, L, a.cap);
+      for (int blk = a.stage_ptr[s] + blockIdx.x; blk < a.stage_ptr[s + 1]; blk += nwg) {
+        ilu_bwd_block<T, T, T, GF, false, BLOCK>(a.F, blk, dst, dst, L, a.cap, threadIdx.x);
+        __syncthreads();
+      }
       ok = grid_sync(a.gb, gen, a.timeout);
     }
   };
@@ -658,7 +832,28 @@ __global__ __launch_bounds__(BLOCK) void k_inner_cg(InnerCgArgs<T> a) {
 // kernel from the producer's per-workgroup partial sums, summed in a fixed
 // order by every workgroup, so no separate reduction launch is needed.
 
-// sum of nb partials (every workgroup computes the same value)
+// q = A p; partial[b] = sum over the workgroup's rows of p_i q_i
+template <typename T, int G>
+__global__ __launch_bounds__(BLOCK) void k_spmv_dot(int64_t nrows, const int64_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const T* __restrict__ val,
+                                                    const T* __restrict__ p, T* __restrict__ q,
+                                                    double* __restrict__ partial) {
+  __shared__ double sh[BLOCK / WAVE];
+  const int lane = threadIdx.x % G;
+  const int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G;
+  double contrib = 0;
+  if (row < nrows) {
+    const T acc = row_dot<T, T, T, G, false>(ci, val, p, rp[row], rp[row + 1], lane, 0, 0);
+    if (lane == 0) {
+      q[row] = acc;
+      contrib = (double)acc * (double)p[row];
+    }
+  }
+  contrib = block_sum(contrib, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = contrib;
+}
+
+// sum of nb partials, identical in every workgroup
 __device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb, double* sh) {
   double s = 0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
@@ -669,56 +864,7 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   return tot;
 }
 
-// q = A p ; partial[b] = sum over the workgroup's rows of p_i q_i
-// (grid-stride over rows so the partial count stays bounded)
-template <typename T, int G>
-__global__ __launch_bounds__(BLOCK) void k_spmv_dot(int64_t nrows, const int64_t* __restrict__ rp,
-                                                    const int32_t* __restrict__ ci, const T* __restrict__ val,
-                                                    const T* __restrict__ p, T* __restrict__ q,
-                                                    double* __restrict__ partial) {
-  __shared__ double sh[BLOCK / WAVE];
-  const int lane = threadIdx.x % G;
-  const int64_t rows_per_pass = (int64_t)gridDim.x * (BLOCK / G);
-  double contrib = 0;
-  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < nrows; row += rows_per_pass) {
-    const int64_t p0 = rp[row], p1 = rp[row + 1];
-    T acc = 0;
-    int64_t k = p0 + lane;
-    for (; k + G < p1; k += 2 * G) {
-      const int32_t c0 = ld_nt(ci + k), c1 = ld_nt(ci + k + G);
-      const T v0 = ld_nt(val + k), v1 = ld_nt(val + k + G);
-      acc += v0 * p[c0];
-      acc += v1 * p[c1];
-    }
-    if (k < p1) acc += ld_nt(val + k) * p[ld_nt(ci + k)];
-    acc = group_reduce<G>(acc);
-    if (lane == 0) {
-      q[row] = acc;
-      contrib += (double)acc * (double)p[row];
-    }
-  }
-  contrib = block_sum(contrib, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = contrib;
-}
-
-// alpha = rz / sum(pq partials); x += alpha p; r -= alpha q
-template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_pcg_update(int64_t n, const double* __restrict__ pq_part, int nb,
-                                                      const double* __restrict__ rz, const T* __restrict__ p,
-                                                      const T* __restrict__ q, T* __restrict__ x,
-                                                      T* __restrict__ r, int first) {
-  __shared__ double sh[BLOCK / WAVE];
-  const double pq = sum_partials(pq_part, nb, sh);
-  const double a = pq != 0.0 ? rz[0] / pq : 0.0;
-  const T alpha = (T)a;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-    const T pi = p[i];
-    x[i] = first ? alpha * pi : x[i] + alpha * pi;
-    r[i] -= alpha * q[i];
-  }
-}
-
-// partial[b] = sum r_i z_i
+// partial[b] = sum a_i b_i
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_dot_partial_t(int64_t n, const T* __restrict__ a, const T* __restrict__ b,
                                                          double* __restrict__ partial) {
@@ -730,8 +876,23 @@ __global__ __launch_bounds__(BLOCK) void k_dot_partial_t(int64_t n, const T* __r
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// rz_new = sum(rz partials); beta = rz_new / rz (0 on the first step);
-// p = z + beta p; rz <- rz_new (workgroup 0 writes after every workgroup read)
+// alpha = rz / pq; x += alpha p (x = alpha p on the first step); r -= alpha q
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(int64_t n, const double* __restrict__ pq,
+                                                      const double* __restrict__ rz, const T* __restrict__ p,
+                                                      const T* __restrict__ q, T* __restrict__ x,
+                                                      T* __restrict__ r, int first) {
+  const double a = pq[0] != 0.0 ? rz[0] / pq[0] : 0.0;
+  const T alpha = (T)a;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const T pi = p[i];
+    x[i] = first ? alpha * pi : x[i] + alpha * pi;
+    r[i] -= alpha * q[i];
+  }
+}
+
+// rz_new = sum(rz partials); beta = rz_new / rz_old (0 on the first step);
+// p = z + beta p; workgroup 0 stores rz_new (another slot than rz_old)
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_pcg_direction(int64_t n, const double* __restrict__ rz_part, int nb,
                                                          const double* __restrict__ rz_old,

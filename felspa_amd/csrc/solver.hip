@@ -179,13 +179,14 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   if (!schedule_is_valid(F, sched))
     throw FsError(FS_ERR_UNSUPPORTED, "ILU fill joins two blocks of one stage; use a larger block size");
   nnz = F.nnz();
-  std::vector<int32_t> ol(n), od(n), ou(n), lp(n), up(n);
+  std::vector<int32_t> ol(n), od(n), ou(n);
+  std::vector<int64_t> lc(n + 1, 0), uc(n + 1, 0);
   int64_t sum_lext = 0, sum_uext = 0;
-  cap = 0;
+  int64_t cap_need = 0;
   max_len = 0;
+  // per-row offsets, and per-column counts of in-block entries
   for (int b = 0; b < sched.n_blocks(); ++b) {
     const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
-    int32_t lacc = 0, uacc = 0;
     for (int32_t r = r0; r < r1; ++r) {
       const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
       const int32_t* c = F.ci.data() + base;
@@ -197,50 +198,102 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       ol[r] = olint;
       od[r] = odiag;
       ou[r] = ouext;
-      lp[r] = lacc;
-      up[r] = uacc;
-      lacc += odiag - olint;
-      uacc += ouext - odiag - 1;
+      for (int32_t p = olint; p < odiag; ++p) lc[c[p] + 1]++;
+      for (int32_t p = odiag + 1; p < ouext; ++p) uc[c[p] + 1]++;
       sum_lext += olint;
       sum_uext += len - ouext;
       max_len = std::max<int>(max_len, (int)len);
     }
-    cap = std::max<int>(cap, std::max(lacc, uacc));
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    lc[i + 1] += lc[i];
+    uc[i + 1] += uc[i];
+  }
+  n_lin = lc[n];
+  n_uin = uc[n];
+  for (int b = 0; b < sched.n_blocks(); ++b) {
+    const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+    cap_need = std::max<int64_t>(cap_need, std::max(lc[r1] - lc[r0], uc[r1] - uc[r0]));
+  }
+  // column-ordered lists: rows visited in ascending order fill each column in
+  // ascending row order
+  std::vector<uint16_t> lr(std::max<int64_t>(n_lin, 1)), ur(std::max<int64_t>(n_uin, 1));
+  std::vector<int64_t> lpo(std::max<int64_t>(n_lin, 1)), upo(std::max<int64_t>(n_uin, 1));
+  {
+    std::vector<int64_t> fl(lc.begin(), lc.end() - 1), fu(uc.begin(), uc.end() - 1);
+    for (int b = 0; b < sched.n_blocks(); ++b) {
+      const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+      for (int32_t r = r0; r < r1; ++r) {
+        const int64_t base = F.rp[r];
+        for (int32_t p = ol[r]; p < od[r]; ++p) {
+          const int32_t c = F.ci[base + p];
+          lr[fl[c]] = (uint16_t)(r - r0);
+          lpo[fl[c]++] = base + p;
+        }
+        for (int32_t p = od[r] + 1; p < ou[r]; ++p) {
+          const int32_t c = F.ci[base + p];
+          ur[fu[c]] = (uint16_t)(r - r0);
+          upo[fu[c]++] = base + p;
+        }
+      }
+    }
   }
   max_rows = std::max(1, (int)sched.max_block_rows);
-  // LDS budget: 4 waves per workgroup, <= 64 KiB per workgroup
-  const int per_wave_budget = 16 * 1024;
+  if (max_rows > 65535) throw FsError(FS_ERR_UNSUPPORTED, "ILU block larger than 65535 rows");
+  // LDS budget: one block per workgroup, <= 48 KiB so several workgroups fit a CU
+  const int per_wave_budget = 48 * 1024;
   const int fixed = (int)lds_bytes_per_wave(max_rows, 0, tsz_max);
-  const int cap_fit = std::max(0, (per_wave_budget - fixed) / (tsz_max + 2));
-  cap = std::min(cap, cap_fit);
   if (fixed > per_wave_budget) throw FsError(FS_ERR_UNSUPPORTED, "ILU block too large for LDS");
+  const int cap_fit = std::max(0, (per_wave_budget - fixed) / (tsz_max + 2));
+  // at least one slot, so the clamped prefetch of an empty column stays inside LDS
+  cap = (int)std::max<int64_t>(1, std::min<int64_t>(cap_need, cap_fit));
+  // lanes per row in the part of a block solve that reads other blocks: the
+  // whole block in one pass of a 256-thread workgroup when it has few rows,
+  // about four entries per lane otherwise
   const double avg = n ? (double)(sum_lext + sum_uext) / (2.0 * n) : 0;
-  G = avg > 48 ? 64 : avg > 24 ? 32 : avg > 12 ? 16 : avg > 6 ? 8 : 4;
+  int g_rows = 4;
+  while (g_rows < 64 && g_rows * 2 * max_rows <= 256) g_rows *= 2;
+  int g_len = 4;
+  while (g_len < 64 && g_len * 4 < avg) g_len *= 2;
+  G = std::max(4, std::min(g_rows, g_len));
   rp.upload(F.rp.data(), 8 * (n + 1));
-  ci.upload(F.ci.data(), 4 * nnz);
-  v64.alloc(8 * nnz);
+  ci.upload(F.ci.data(), 4 * std::max<int64_t>(1, nnz));
+  v64.alloc(8 * std::max<int64_t>(1, nnz));
   o_lint.upload(ol.data(), 4 * n);
   o_diag.upload(od.data(), 4 * n);
   o_uext.upload(ou.data(), 4 * n);
-  lpre.upload(lp.data(), 4 * n);
-  upre.upload(up.data(), 4 * n);
   blk_ptr.upload(sched.blk_ptr.data(), 4 * sched.blk_ptr.size());
   stage_ptr.upload(sched.stage_ptr.data(), 4 * sched.stage_ptr.size());
+  lcp.upload(lc.data(), 8 * (n + 1));
+  ucp.upload(uc.data(), 8 * (n + 1));
+  lrow.upload(lr.data(), 2 * lr.size());
+  urow.upload(ur.data(), 2 * ur.size());
+  lpos.upload(lpo.data(), 8 * lpo.size());
+  upos.upload(upo.data(), 8 * upo.size());
+  lv64.alloc(8 * lpo.size());
+  uv64.alloc(8 * upo.size());
+  lv32.alloc(4 * lpo.size());
+  uv32.alloc(4 * upo.size());
   host_pattern = std::move(F);
 }
 
 template <typename T>
 IluView<T> IluDev::view() const {
+  const bool d = std::is_same<T, double>::value;
   IluView<T> v;
   v.rp = (const int64_t*)rp.p;
   v.ci = (const int32_t*)ci.p;
-  v.v = std::is_same<T, double>::value ? (T*)v64.p : (T*)v32.p;
+  v.v = d ? (T*)v64.p : (T*)v32.p;
   v.o_lint = (const int32_t*)o_lint.p;
   v.o_diag = (const int32_t*)o_diag.p;
   v.o_uext = (const int32_t*)o_uext.p;
-  v.lpre = (const int32_t*)lpre.p;
-  v.upre = (const int32_t*)upre.p;
   v.blk_ptr = (const int32_t*)blk_ptr.p;
+  v.lcp = (const int64_t*)lcp.p;
+  v.lrow = (const uint16_t*)lrow.p;
+  v.lv = d ? (const T*)lv64.p : (const T*)lv32.p;
+  v.ucp = (const int64_t*)ucp.p;
+  v.urow = (const uint16_t*)urow.p;
+  v.uv = d ? (const T*)uv64.p : (const T*)uv32.p;
   return v;
 }
 template IluView<double> IluDev::view<double>() const;
@@ -265,19 +318,33 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
   }
   cnt.bytes += (double)nnz * 12.0 * 2.0;
   hip_check(hipGetLastError(), "ilu numeric launch");
-  if (!v32.p) v32.alloc(4 * nnz);
+  if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
   hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz,
                      (const double*)v64.p, (float*)v32.p);
   cnt.launches++;
+  gather_inblock(st);
+}
+
+// in-block entries in column order, both precisions
+void IluDev::gather_inblock(hipStream_t st) {
+  if (n_lin > 0) {
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(n_lin)), dim3(BLOCK), 0, st, n_lin, lpos.as<int64_t>(),
+                       v64.as<double>(), lv64.as<double>(), lv32.as<float>());
+  }
+  if (n_uin > 0) {
+    hipLaunchKernelGGL(k_gather_vals, dim3(grid_for(n_uin)), dim3(BLOCK), 0, st, n_uin, upos.as<int64_t>(),
+                       v64.as<double>(), uv64.as<double>(), uv32.as<float>());
+  }
 }
 
 template <typename T, typename TB, typename TY, int G, bool UNIT, bool FWD>
 static void launch_stage(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
-  const size_t per_wave = lds_bytes_per_wave(F.max_rows, F.cap, sizeof(T));
+  const size_t lds = lds_bytes_per_wave(F.max_rows, F.cap, sizeof(T));
   const int nb = b1 - b0;
-  const int g = std::max(1, std::min((nb + 3) / 4, 16384));
-  hipLaunchKernelGGL((k_ilu_stage<T, TB, TY, G, UNIT, FWD>), dim3(g), dim3(BLOCK), per_wave * 4, st,
-                     F.view<T>(), b0, b1, b, y, F.max_rows, F.cap);
+  if (nb <= 0) return;
+  const int g = std::max(1, std::min(nb, 65535));
+  hipLaunchKernelGGL((k_ilu_stage<T, TB, TY, G, UNIT, FWD>), dim3(g), dim3(BLOCK), lds, st, F.view<T>(), b0, b1,
+                     b, y, F.max_rows, F.cap);
 }
 
 template <typename T, typename TB, typename TY, bool UNIT, bool FWD>
@@ -343,18 +410,86 @@ void spmv_launch(int G, int64_t nrows, int64_t row_off, const int64_t* rp, const
 #undef FS_SPMV_CASE
 }
 
+// lanes per row so that each lane streams about four entries
 int lanes_for(double avg_row) {
-  return avg_row > 96 ? 64 : avg_row > 48 ? 32 : avg_row > 20 ? 16 : avg_row > 8 ? 8 : 4;
+  return avg_row > 192 ? 64 : avg_row > 96 ? 32 : avg_row > 48 ? 16 : avg_row > 20 ? 8 : 4;
 }
 
-void DevCsr::upload_pattern(const HostCsr& h) {
+void DevCsr::upload_pattern(const HostCsr& h, bool with_padded) {
   n_rows = h.n_rows;
   n_cols = h.n_cols;
   nnz = h.nnz();
   rp.upload(h.rp.data(), 8 * (n_rows + 1));
   ci.upload(h.ci.data(), 4 * std::max<int64_t>(nnz, 1));
   G = lanes_for(n_rows ? (double)nnz / n_rows : 1);
+  padded = with_padded;
+  if (!padded) return;
+  std::vector<int64_t> r4(n_rows + 1, 0);
+  for (int64_t r = 0; r < n_rows; ++r) r4[r + 1] = r4[r] + (h.rp[r + 1] - h.rp[r] + 3) / 4;
+  nnz4 = r4[n_rows];
+  std::vector<int32_t> c4(4 * std::max<int64_t>(nnz4, 1), 0);
+  pad_pos.assign(nnz, 0);
+  for (int64_t r = 0; r < n_rows; ++r) {
+    const int64_t b = h.rp[r], e = h.rp[r + 1];
+    int64_t o = 4 * r4[r];
+    for (int64_t p = b; p < e; ++p) {
+      c4[o] = h.ci[p];
+      pad_pos[p] = o++;
+    }
+    const int32_t fill = e > b ? h.ci[e - 1] : 0;  // padding reads a column the row already uses
+    for (; o < 4 * r4[r + 1]; ++o) c4[o] = fill;
+  }
+  rp4.upload(r4.data(), 8 * (n_rows + 1));
+  ci4.upload(c4.data(), 4 * c4.size());
+  v4_64.alloc(8 * c4.size());
+  v4_32.alloc(4 * c4.size());
+  pad_host.assign(c4.size(), 0.0);
+  const double g = n_rows ? (double)nnz4 / n_rows : 1;  // groups per row, about two per lane
+  G4 = g > 64 ? 32 : g > 32 ? 16 : g > 12 ? 8 : 4;
 }
+
+void DevCsr::upload_values(const double* vals, hipStream_t st) {
+  v64.upload(vals, 8 * std::max<int64_t>(1, nnz));
+  if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, v64.as<double>(),
+                     v32.as<float>());
+  if (padded) {
+    for (int64_t e = 0; e < nnz; ++e) pad_host[pad_pos[e]] = vals[e];
+    v4_64.upload(pad_host.data(), 8 * pad_host.size());
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad_host.size())), dim3(BLOCK), 0, st,
+                       (int64_t)pad_host.size(), v4_64.as<double>(), v4_32.as<float>());
+  }
+  hip_check(hipStreamSynchronize(st), "sync");
+}
+
+// padded SpMV launch (rows [row_off, row_off + nrows))
+template <typename TV, typename TX, typename TY, typename TB, typename TACC, int MODE>
+void spmv4_launch(const DevCsr& M, int64_t nrows, int64_t row_off, const TX* x, TY* y, const TB* b,
+                  int64_t col_limit, hipStream_t st) {
+  const int G = M.G4;
+  const int rows_per_block = BLOCK / G;
+  const int g = (int)std::max<int64_t>(1, (nrows + rows_per_block - 1) / rows_per_block);
+  using V4 = typename Vec4<TV>::type;
+  const V4* vv = std::is_same<TV, double>::value ? (const V4*)M.v4_64.p : (const V4*)M.v4_32.p;
+#define FS_SPMV4_CASE(GG)                                                                                      \
+  case GG:                                                                                                     \
+    hipLaunchKernelGGL((k_spmv4<TV, TX, TY, TB, TACC, GG, MODE>), dim3(g), dim3(BLOCK), 0, st, nrows, row_off, \
+                       M.rp4.as<int64_t>(), M.ci4.as<i32x4>(), vv, x, y, b, col_limit);                        \
+    break;
+  switch (G) {
+    FS_SPMV4_CASE(4)
+    FS_SPMV4_CASE(8)
+    FS_SPMV4_CASE(16)
+    default:
+      FS_SPMV4_CASE(32)
+  }
+#undef FS_SPMV4_CASE
+}
+template void spmv4_launch<double, double, double, double, double, 0>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<double, double, double, double, double, 1>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<double, double, double, double, double, 2>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<float, float, double, double, float, 0>(const DevCsr&, int64_t, int64_t, const float*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<float, float, float, double, float, 2>(const DevCsr&, int64_t, int64_t, const float*, float*, const double*, int64_t, hipStream_t);
 
 }  // namespace fs
 

@@ -49,15 +49,28 @@ struct DevCsr {
   int64_t n_rows = 0, n_cols = 0, nnz = 0;
   int G = 16;
   DevBuf rp, ci, v64, v32;
-  void upload_pattern(const HostCsr& h);
+  // padded copy for the streaming SpMV: rows start at multiples of 4 entries
+  bool padded = false;
+  int64_t nnz4 = 0;  // groups of 4
+  int G4 = 4;
+  DevBuf rp4, ci4, v4_64, v4_32;
+  std::vector<int64_t> pad_pos;  // entry -> padded position
+  std::vector<double> pad_host;
+  void upload_pattern(const HostCsr& h, bool with_padded = false);
+  // values in the plain layout (host), both precisions and both layouts
+  void upload_values(const double* vals, hipStream_t st);
 };
 
 struct IluDev {
   int64_t n = 0, nnz = 0;
   Schedule sched;
   HostPattern host_pattern;
-  DevBuf rp, ci, v64, v32, o_lint, o_diag, o_uext, lpre, upre, blk_ptr, stage_ptr;
+  DevBuf rp, ci, v64, v32, o_lint, o_diag, o_uext, blk_ptr, stage_ptr;
+  // in-block entries in column order (values gathered after factorisation)
+  DevBuf lcp, lrow, lpos, lv64, lv32, ucp, urow, upos, uv64, uv32;
+  int64_t n_lin = 0, n_uin = 0;
   int max_rows = 1, cap = 0, max_len = 1, G = 16;
+  void gather_inblock(hipStream_t st);
   void build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max);
   template <typename T>
   fsk::IluView<T> view() const;
@@ -72,6 +85,10 @@ struct IluDev {
 template <typename TV, typename TX, typename TY, typename TB, typename TACC, int MODE>
 void spmv_launch(int G, int64_t nrows, int64_t row_off, const int64_t* rp, const int32_t* ci, const TV* v,
                  const TX* x, TY* y, const TB* b, int64_t col_limit, int64_t x_off, hipStream_t st);
+
+template <typename TV, typename TX, typename TY, typename TB, typename TACC, int MODE>
+void spmv4_launch(const DevCsr& M, int64_t nrows, int64_t row_off, const TX* x, TY* y, const TB* b,
+                  int64_t col_limit, hipStream_t st);
 
 HostCsr csr_from_abi(const fs_csr* m, const char* name, bool need_vals);
 std::vector<int64_t> caller_to_sorted(const fs_csr* m);

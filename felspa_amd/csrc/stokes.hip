@@ -42,7 +42,8 @@ constexpr int SC_Y = SC_GAM + MAXM + 1;     // [MAXM]
 constexpr int SC_NRM = SC_Y + MAXM;         // 1
 constexpr int SC_EST = SC_NRM + 1;          // 1
 constexpr int SC_TMP = SC_EST + 1;          // 4
-constexpr int SC_H = SC_TMP + 4;            // [(MAXM+1) * MAXM]
+constexpr int SC_PCG = SC_TMP + 4;          // 4: p.q, r.z (two slots)
+constexpr int SC_H = SC_PCG + 4;            // [(MAXM+1) * MAXM]
 constexpr int SC_TOTAL = SC_H + (MAXM + 1) * MAXM;
 
 struct Ctrl {  // dealii::ReductionControl::check (documented semantics)
@@ -78,7 +79,7 @@ struct fs_handle {
   std::vector<int64_t> ablk_from_k;                       // A entry -> K position
   std::vector<double> hK, hMp, hA;                        // host value staging
   IluDev FA, FS;
-  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
+  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part, tickets;
   int64_t ldu = 0;
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
@@ -115,13 +116,8 @@ struct fs_handle {
 
   // y = K x (FP64) or y = b - K x
   void kmult64(const double* xv, double* y, const double* bv) {
-    if (bv)
-      spmv_launch<double, double, double, double, double, 1>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
-                                                             K.v64.as<double>(), xv, y, bv, 0, 0, st);
-    else
-      spmv_launch<double, double, double, double, double, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
-                                                             K.v64.as<double>(), xv, y, (const double*)nullptr,
-                                                             0, 0, st);
+    if (bv) spmv4_launch<double, double, double, double, double, 1>(K, n, 0, xv, y, bv, 0, st);
+    else spmv4_launch<double, double, double, double, double, 0>(K, n, 0, xv, y, (const double*)nullptr, 0, st);
     cnt.launches++;
     cnt.bytes += K.nnz * 12.0 + 8.0 * (n + 1) + 8.0 * n * (bv ? 3 : 2);
   }
@@ -157,7 +153,7 @@ struct fs_handle {
     a.fail_out = dptr<int>(flags, 2);
     a.gb = dptr<GridBarrier>(gbar);
     a.timeout = timeout_ticks();
-    const size_t lds = lds_bytes_per_wave(FS.max_rows, FS.cap, sizeof(T)) * (BLOCK / WAVE);
+    const size_t lds = lds_bytes_per_wave(FS.max_rows, FS.cap, sizeof(T));
     void* fn;
     const int gm = Mp.G <= 8 ? 8 : Mp.G <= 16 ? 16 : 32;
     fn = pick_cg_kernel<T>(gm, FS.G);
@@ -186,33 +182,34 @@ struct fs_handle {
 
   // BlockSchurPreconditionerBase::vmult (linear_operators.implement.h:200-209):
   // dst_u = ILU_A^{-1} src_u ; t = B dst_u - src_p ; dst_p = CG(M_p, ILU_S; t)
-  // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0
+  // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0.
+  // Scalars never leave the device: the update / direction kernels sum the
+  // producers' per-workgroup partials themselves (fixed order).
   template <typename T>
   void inner_pcg_A(const double* src, T* x, int iters) {
     T* base = (T*)pcg.p;
     T *r = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
     double* part_pq = dptr<double>(pcg_part);
-    double* part_rz = part_pq + 1024;
-    double* rz = part_pq + 2048;  // two slots
-    const int gv = 512;
-    const int gs = std::min<int>(1024, (int)std::max<int64_t>(1, (nu * Ablk.G + BLOCK - 1) / BLOCK));
-    const T* av = std::is_same<T, double>::value ? (const T*)Ablk.v64.p : (const T*)Ablk.v32.p;
+    double* part_rz = part_pq + spmv_dot_grid();
+    double* rz = S(SC_PCG + 1);  // two slots
+    const int gv = 512, gu = 256;
     hipLaunchKernelGGL((k_copy_convert<double, T>), dim3(gv), dim3(BLOCK), 0, st, nu, src, r);
     FA.apply<T, T>(r, y, z, st, cnt);
     hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
-    hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
+    hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
     cnt.launches += 3;
     int cur = 0;
     for (int it = 0; it < iters; ++it) {
-      launch_spmv_dot<T>(gs, av, pv, q, part_pq);
-      hipLaunchKernelGGL((k_pcg_update<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_pq, gs, rz + cur, pv, q, x, r,
+      launch_spmv_dot<T>(pv, q, part_pq);
+      hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
+      hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
                          it == 0 ? 1 : 0);
-      cnt.launches += 2;
+      cnt.launches += 3;
       cnt.bytes += Ablk.nnz * (sizeof(T) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
       if (it + 1 == iters) break;
       FA.apply<T, T>(r, y, z, st, cnt);
       hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
-      hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gv), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
+      hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
                          rz + (cur ^ 1), z, pv, 0);
       cnt.launches += 2;
       cnt.bytes += sizeof(T) * 5.0 * nu;
@@ -220,14 +217,23 @@ struct fs_handle {
     }
   }
 
+  int spmv_dot_grid() const { return (int)std::max<int64_t>(1, (nu * Ablk.G4 + BLOCK - 1) / BLOCK); }
+
+  // q = A p and the per-workgroup partials of p.q, on the padded A block;
+  // av selects the precision (Ablk.v4_32 or Ablk.v4_64)
   template <typename T>
-  void launch_spmv_dot(int g, const T* av, const T* pv, T* q, double* part) {
-#define FS_SD(GG) \
-  case GG: hipLaunchKernelGGL((k_spmv_dot<T, GG>), dim3(g), dim3(BLOCK), 0, st, nu, Ablk.rp.as<int64_t>(), \
-                              Ablk.ci.as<int32_t>(), av, pv, q, part); break;
-    switch (Ablk.G) {
-      FS_SD(4) FS_SD(8) FS_SD(16) FS_SD(32)
-      default: FS_SD(64)
+  void launch_spmv_dot(const T* pv, T* q, double* part) {
+    const int g = spmv_dot_grid();
+    using V4 = typename Vec4<T>::type;
+    const V4* av = std::is_same<T, double>::value ? (const V4*)Ablk.v4_64.p : (const V4*)Ablk.v4_32.p;
+#define FS_SD(GG)                                                                                            \
+  case GG:                                                                                                   \
+    hipLaunchKernelGGL((k_spmv4_dot<T, GG>), dim3(g), dim3(BLOCK), 0, st, nu, Ablk.rp4.as<int64_t>(),         \
+                       Ablk.ci4.as<i32x4>(), av, pv, q, part);                                              \
+    break;
+    switch (Ablk.G4) {
+      FS_SD(4) FS_SD(8) FS_SD(16)
+      default: FS_SD(32)
     }
 #undef FS_SD
   }
@@ -238,9 +244,7 @@ struct fs_handle {
     const int ia = o.method == FS_METHOD_GMRES_REF ? 0 : o.inner_a_iters;
     if (ia > 0) inner_pcg_A<T>(src, dst, ia);
     else FA.apply<T, double>(src, y, dst, st, cnt);
-    const T* kv = std::is_same<T, double>::value ? (const T*)K.v64.p : (const T*)K.v32.p;
-    spmv_launch<T, T, T, double, T, 2>(K.G, np, nu, K.rp.as<int64_t>(), K.ci.as<int32_t>(), kv, dst, (T*)tp.p,
-                                       src + nu, nu, 0, st);
+    spmv4_launch<T, T, T, double, T, 2>(K, np, nu, dst, (T*)tp.p, src + nu, nu, st);
     cnt.launches++;
     inner_cg<T>((const T*)tp.p, dst + nu);
   }
@@ -308,14 +312,7 @@ struct fs_handle {
         T* zj = Zb + (int64_t)j * ld;
         precond<T>(Vb + (int64_t)j * ld, zj);
         double* w = Vb + (int64_t)(j + 1) * ld;
-        if (std::is_same<T, double>::value)
-          spmv_launch<double, double, double, double, double, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
-                                                                 K.v64.as<double>(), (const double*)zj, w,
-                                                                 (const double*)nullptr, 0, 0, st);
-        else
-          spmv_launch<float, float, double, double, float, 0>(K.G, n, 0, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
-                                                              K.v32.as<float>(), (const float*)zj, w,
-                                                              (const double*)nullptr, 0, 0, st);
+        spmv4_launch<T, T, double, double, T, 0>(K, n, 0, (const T*)zj, w, (const double*)nullptr, 0, st);
         cnt.launches++;
         cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
         arnoldi_tail(j);
@@ -383,20 +380,10 @@ struct fs_handle {
   }
 
   void upload_values() {
-    K.v64.upload(hK.data(), 8 * K.nnz);
-    if (!K.v32.p) K.v32.alloc(4 * std::max<int64_t>(1, K.nnz));
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(BLOCK), 0, st, K.nnz, K.v64.as<double>(),
-                       K.v32.as<float>());
+    K.upload_values(hK.data(), st);
     for (size_t e = 0; e < ablk_from_k.size(); ++e) hA[e] = hK[ablk_from_k[e]];
-    Ablk.v64.upload(hA.data(), 8 * Ablk.nnz);
-    if (!Ablk.v32.p) Ablk.v32.alloc(4 * std::max<int64_t>(1, Ablk.nnz));
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(4096), dim3(BLOCK), 0, st, Ablk.nnz, Ablk.v64.as<double>(),
-                       Ablk.v32.as<float>());
-    Mp.v64.upload(hMp.data(), 8 * Mp.nnz);
-    if (!Mp.v32.p) Mp.v32.alloc(4 * std::max<int64_t>(1, Mp.nnz));
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(1024), dim3(BLOCK), 0, st, Mp.nnz, Mp.v64.as<double>(),
-                       Mp.v32.as<float>());
-    sync();
+    Ablk.upload_values(hA.data(), st);
+    Mp.upload_values(hMp.data(), st);
     dirty = true;
   }
 
@@ -461,7 +448,7 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
       }
     }
   }
-  h.K.upload_pattern(Kh);
+  h.K.upload_pattern(Kh, true);
   h.hK.assign(Kh.nnz(), 0.0);
   // A alone (velocity rows, velocity columns) in solver order
   HostCsr Ah;
@@ -479,7 +466,7 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
     for (int64_t q = Kh.rp[r]; q < Kh.rp[r + 1]; ++q)
       if (Kh.ci[q] < nu) { Ah.ci[o] = Kh.ci[q]; h.ablk_from_k[o] = q; ++o; }
   }
-  h.Ablk.upload_pattern(Ah);
+  h.Ablk.upload_pattern(Ah, true);
   h.hA.assign(Ah.nnz(), 0.0);
 }
 
@@ -651,8 +638,9 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.vin.alloc(8 * h.ld);
   h.ldu = (h.nu + 63) & ~int64_t(63);
   h.pcg.alloc(8 * 5 * h.ldu);
-  h.pcg_part.alloc(8 * 4096);
-  hip_check(hipMemset(h.pcg_part.p, 0, 8 * 4096), "memset");
+  h.pcg_part.alloc(8 * (size_t)(h.spmv_dot_grid() + 1024));
+  h.tickets.alloc(64);
+  hip_check(hipMemset(h.tickets.p, 0, 64), "memset");
   h.partial.alloc(8 * RED_BLOCKS * MAXM);
   h.sc.alloc(8 * SC_TOTAL);
   h.gbar.alloc(sizeof(GridBarrier));
@@ -760,9 +748,8 @@ int fs_stokes_vmult(fs_handle* h, const double* src, double* dst, int32_t precis
   if (precision == FS_PREC_MIXED) {
     hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, n, h->b.as<double>(),
                        (float*)h->ytmp.p);
-    spmv_launch<float, float, double, double, float, 0>(h->K.G, n, 0, h->K.rp.as<int64_t>(), h->K.ci.as<int32_t>(),
-                                                        h->K.v32.as<float>(), (const float*)h->ytmp.p,
-                                                        h->r.as<double>(), (const double*)nullptr, 0, 0, h->st);
+    spmv4_launch<float, float, double, double, float, 0>(h->K, n, 0, (const float*)h->ytmp.p, h->r.as<double>(),
+                                                         (const double*)nullptr, 0, h->st);
   } else {
     h->kmult64(h->b.as<double>(), h->r.as<double>(), nullptr);
   }
@@ -845,16 +832,16 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
   hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, h->n, h->vin.as<double>(),
                      (float*)h->ytmp.p);
   void* base = h->pcg.p;
-  double* part = dptr<double>(h->pcg_part);
   auto once = [&]() {
     h->cnt = Counters{};
     switch (kind) {
       case 0: {
-        const int gs = std::min<int>(1024, (int)std::max<int64_t>(1, (nu * h->Ablk.G + BLOCK - 1) / BLOCK));
         if (mixed)
-          h->launch_spmv_dot<float>(gs, h->Ablk.v32.as<float>(), (const float*)h->ytmp.p, (float*)base, part);
+          h->launch_spmv_dot<float>((const float*)h->ytmp.p, (float*)base,
+                                    dptr<double>(h->pcg_part));
         else
-          h->launch_spmv_dot<double>(gs, h->Ablk.v64.as<double>(), h->vin.as<double>(), (double*)base, part);
+          h->launch_spmv_dot<double>(h->vin.as<double>(), (double*)base,
+                                     dptr<double>(h->pcg_part));
         h->cnt.bytes = h->Ablk.nnz * ((mixed ? 4.0 : 8.0) + 4.0) + 8.0 * (nu + 1) + (mixed ? 4.0 : 8.0) * 2.0 * nu;
         break;
       }
