@@ -26,9 +26,9 @@ WORKLOADS = {
           "viscosity 1/100, density 1/1.1, unit gravity, free slip; ILU(1) on A, ILU(0) on M_p",
     "C1": "2D inclusion, Q2-Q1 on 64x64 quads, viscosity contrast 1e3 disk r=0.1, simple shear, "
           "periodic x; ILU(0)",
-    "C3": "3D porphyroclast, Q2-Q1 on 32^3 hexes, sphere r=0.2 contrast 1e3, linear simple shear; "
-          "ILU(1)",
-    "C4": "2D 128x128 seeded ellipses, simple shear; ILU(1)",
+    "C3": "3D porphyroclast, Q2-Q1 on 32^3 hexes, sphere r=0.2 contrast 1e3, linear simple shear",
+    "C4": "2D 128x128 seeded ellipses (1-8, log10 contrast U[0,3]), simple shear",
+    "C5": "3D 48^3 seeded ellipsoids (1-4, log10 contrast U[1,3]), simple shear",
 }
 PEAK_HBM_GBS = 8000.0
 
@@ -46,7 +46,8 @@ def main():
     ap.add_argument("--n-cells", type=int, default=None)
     ap.add_argument("--inner-a", type=int, default=60)
     ap.add_argument("--restart", type=int, default=30)
-    ap.add_argument("--hint", type=int, default=5)
+    ap.add_argument("--hint", type=int, default=None, help="lattice nodes per block edge (default 5 in 2D, 3 in 3D)")
+    ap.add_argument("--ilu-k", type=int, default=None, help="override the configuration's ILU level on A")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")
@@ -63,10 +64,14 @@ def main():
     cfg = args.config
     settings = fa.solver_settings(cfg)
     kA, kS = settings["ilu_k_A"], settings["ilu_k_S"]
+    if args.ilu_k is not None:
+        kA = args.ilu_k
     t0 = time.time()
     S = fa.StokesSystem(fa.make_problem(cfg, n_cells=args.n_cells))
     log("assembled %s: n_u=%d n_p=%d nnz(A)=%d in %.1fs" % (cfg, S.n_u, S.n_p, S.blocks["A"].nnz,
                                                             time.time() - t0))
+    if args.hint is None:
+        args.hint = 5 if S.dim == 2 else 3
     hint = S.block_hint(args.hint)
     common = dict(A_lof=kA, S_lof=kS, block_hint=hint, constraints=S.constrained,
                   restart=args.restart, inner_a_iters=args.inner_a)
@@ -183,7 +188,7 @@ def main():
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": "synthetic (generated Q2-Q1 system)",
         "config": {"workload": WORKLOADS.get(cfg, cfg), "config_id": cfg, "n_u": S.n_u, "n_p": S.n_p,
-                   "nnz_A": int(S.blocks["A"].nnz), "restart": args.restart, "inner_a_pcg": args.inner_a,
+                   "nnz_A": int(S.blocks["A"].nnz), "ilu_k_A": kA, "ilu_k_S": kS, "restart": args.restart, "inner_a_pcg": args.inner_a,
                    "block_nodes": args.hint, "guess": "zero", "learned_guess": False},
         "outer_iters": st_mixed["outer_iters"], "inner_iters": n_inner,
         "true_rel_residual": st_mixed["true_rel_residual"],

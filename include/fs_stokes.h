@@ -115,9 +115,10 @@ int fs_stokes_refactor(fs_handle* h, const double* A_val, const double* Bt_val,
                        const double* B_val, const double* K11_val, const double* Mp_val);
 
 /* StokesLinearSystem::solve (stokes_ilu.cc:442-468 -> solve_gmres :509-561):
- * rhs [n_u+n_p] host, x [n_u+n_p] host: guess in, solution out. Constrained
- * entries of the guess are zeroed first (stokes_ilu.cc:523-524). Returns
- * FS_OK, FS_NOT_CONVERGED or an error. */
+ * rhs [n_u+n_p] host, x [n_u+n_p] host: guess in, solution out. The caller
+ * zeroes the constrained entries of the guess before the call and distributes
+ * the constraints after it, as the reference does around its GMRES call
+ * (stokes_ilu.cc:523-524, :545). Returns FS_OK, FS_NOT_CONVERGED or an error. */
 int fs_stokes_solve(fs_handle* h, const double* rhs, double* x, fs_stats* stats);
 
 /* Same, with rhs and x already resident in device memory (FP64). */

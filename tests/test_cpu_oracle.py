@@ -171,3 +171,16 @@ def test_reference_mode_stops_on_reduction(oracle):
     assert st["status"] == 0
     x2, st2 = oracle.stokes_solve(S.blocks, S.rhs, kA=1, kS=0, max_steps=3)
     assert st2["status"] == 1 and st2["outer_iters"] == 3
+
+
+def test_spmv_against_dense(oracle):
+    """or_spmv (deal.II SparseMatrix::vmult, diagonal first) = dense M x, on a
+    square block and on a rectangular one."""
+    S = system("C1", 4)
+    x = np.random.default_rng(7).standard_normal(S.n_u)
+    for name in ("A", "B"):
+        M = S.blocks[name]
+        xv = x[:M.shape[1]]
+        y = oracle.spmv(M, xv)
+        want = M.toarray() @ xv
+        assert np.abs(y - want).max() <= 1e-13 * np.abs(M).sum(axis=1).max() * np.abs(xv).max()
