@@ -41,7 +41,8 @@ class FsStats(C.Structure):
     _fields_ = [("status", i32), ("outer_iters", i32), ("restarts", i32), ("guess_used", i32),
                 ("inner_iters", i64), ("last_value", f64), ("initial_rel_residual", f64),
                 ("true_rel_residual", f64), ("t_setup_ms", f64), ("t_krylov_ms", f64),
-                ("t_total_ms", f64), ("bytes_algorithmic", f64), ("n_kernel_launches", i64)]
+                ("t_total_ms", f64), ("bytes_algorithmic", f64), ("n_kernel_launches", i64),
+                ("t_guess_ms", f64), ("guess_rel_residual", f64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -55,6 +56,11 @@ class FsProblem(C.Structure):
                 ("axis_min", f64), ("axis_max", f64), ("logc_min", f64), ("logc_max", f64)]
 
 
+class FsGuessConfig(C.Structure):
+    _fields_ = [("dim", i32), ("width", i32), ("modes", i32), ("layers", i32), ("proj", i32),
+                ("seed", C.c_uint64)]
+
+
 class FsSystemInfo(C.Structure):
     _fields_ = [("dim", i32), ("n_cells", i32), ("n_u", i64), ("n_p", i64), ("nnz_A", i64),
                 ("nnz_Bt", i64), ("nnz_B", i64), ("nnz_K11", i64), ("nnz_Mp", i64),
@@ -65,13 +71,19 @@ class FsSystemInfo(C.Structure):
 EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
-    "fs_stokes_set_guess_field", "fs_stokes_get_ordering", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
+    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_get_ordering", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_Tvmult", "fs_ilu_get_factors",
     "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
     "fs_system_create", "fs_system_destroy", "fs_system_get_info", "fs_system_csr",
     "fs_system_rhs", "fs_system_constraints", "fs_system_distribute", "fs_system_dof_maps",
     "fs_system_log_eta_grid", "fs_system_block_hint",
+    "fs_guess_model_create", "fs_guess_model_destroy", "fs_guess_model_get_config",
+    "fs_guess_model_save", "fs_guess_model_load", "fs_guess_model_num_params",
+    "fs_guess_model_get_params", "fs_guess_model_set_params", "fs_guess_model_get_grads",
+    "fs_guess_model_set_normalization", "fs_guess_model_get_normalization", "fs_guess_prepare_input",
+    "fs_guess_model_forward", "fs_guess_model_loss_grad", "fs_guess_model_adam_step", "fs_gemm_bf16",
+    "fs_guess_model_time_forward",
 ]
 
 _lib = None
@@ -95,6 +107,7 @@ def lib():
         "fs_stokes_vmult": (i32, [V, P(f64), P(f64), i32]),
         "fs_stokes_precond_vmult": (i32, [V, P(f64), P(f64), i32, P(FsStats)]),
         "fs_stokes_set_guess_field": (i32, [V, V, P(f32), i32, P(i64), P(C.c_int8), P(i64)]),
+        "fs_stokes_set_guess_mode": (i32, [V, i32]),
         "fs_stokes_get_ordering": (i32, [V, i32, P(i32), P(i32)]),
         "fs_stokes_time_kernel": (i32, [V, i32, i32, i32, P(f64), P(f64)]),
         "fs_last_error": (C.c_char_p, [V]),
@@ -123,6 +136,24 @@ def lib():
         "fs_system_dof_maps": (i32, [V, P(P(i64)), P(P(C.c_int8)), P(P(i64))]),
         "fs_system_log_eta_grid": (P(f32), [V]),
         "fs_system_block_hint": (i32, [V, i32, P(i32)]),
+        "fs_guess_model_create": (i32, [P(FsGuessConfig), P(V)]),
+        "fs_guess_model_destroy": (None, [V]),
+        "fs_guess_model_get_config": (i32, [V, P(FsGuessConfig)]),
+        "fs_guess_model_save": (i32, [V, C.c_char_p]),
+        "fs_guess_model_load": (i32, [C.c_char_p, P(V)]),
+        "fs_guess_model_num_params": (i64, [V]),
+        "fs_guess_model_get_params": (i32, [V, P(f32)]),
+        "fs_guess_model_set_params": (i32, [V, P(f32)]),
+        "fs_guess_model_get_grads": (i32, [V, P(f32)]),
+        "fs_guess_model_set_normalization": (i32, [V, P(f32), P(f32)]),
+        "fs_guess_model_get_normalization": (i32, [V, P(f32), P(f32)]),
+        "fs_guess_prepare_input": (i32, [V, i32, i32, V, V]),
+        "fs_guess_model_forward": (i32, [V, i32, i32, V, V]),
+        "fs_guess_model_loss_grad": (i32, [V, i32, i32, V, V, P(f64)]),
+        "fs_guess_model_adam_step": (i32, [V, f32, f32, f32, f32, f32]),
+        "fs_gemm_bf16": (i32, [i32, i32, i32, i32, V, i32, i64, i64, i64, V, i32, i64, i64, i64,
+                               V, i64, i64, i64, f32, f32]),
+        "fs_guess_model_time_forward": (i32, [V, i32, i32, i32, P(f64), P(f64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

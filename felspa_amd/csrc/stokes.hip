@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "fs_stokes.h"
+#include "guess_internal.h"
 #include "solver_internal.h"
 
 using namespace fsk;
@@ -68,6 +69,32 @@ T* dptr(const DevBuf& b, int64_t off = 0) {
   return (T*)b.p + off;
 }
 
+struct GuessNorm {
+  float mean[4], stdv[4];
+  int64_t P;  // lattice points per output channel
+};
+
+// x[r] = network output at row r's lattice node and channel, denormalised
+__global__ __launch_bounds__(BLOCK) void k_guess_scatter(int64_t n, const int32_t* __restrict__ idx,
+                                                         const float* __restrict__ Y, GuessNorm gn,
+                                                         double* __restrict__ x) {
+  for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < n; r += (int64_t)gridDim.x * BLOCK) {
+    const int32_t id = idx[r];
+    const int ch = (int)(id / gn.P);
+    x[r] = (double)(Y[id] * gn.stdv[ch] + gn.mean[ch]);
+  }
+}
+
+// rows holding only their diagonal (constrained dofs): x = b / K_rr
+__global__ __launch_bounds__(BLOCK) void k_diag_rows(int64_t nr, const int32_t* __restrict__ rows,
+                                                     const int64_t* __restrict__ rp, const double* __restrict__ kv,
+                                                     const double* __restrict__ b, double* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nr; i += (int64_t)gridDim.x * BLOCK) {
+    const int32_t r = rows[i];
+    x[r] = b[r] / kv[rp[r]];
+  }
+}
+
 }  // namespace
 
 struct fs_handle {
@@ -84,12 +111,11 @@ struct fs_handle {
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
   Counters cnt;
-  // learned guess
+  // learned guess (FS_GUESS_LEARNED)
   fs_guess_model* model = nullptr;
-  std::vector<float> guess_field;
-  int guess_ncells = 0;
-  std::vector<int64_t> g_vnode, g_pnode;
-  std::vector<int8_t> g_vcomp;
+  int guess_nc = 0;
+  DevBuf g_logeta, g_idx, g_diag_rows;
+  std::vector<int32_t> diag_only;  // solver rows whose K row is the diagonal alone
 
   ~fs_handle() {
     if (h_pin) (void)hipHostFree(h_pin);
@@ -396,6 +422,45 @@ struct fs_handle {
     upload_values();
   }
 
+  // network guess into x (solver order); falls back to zero when its
+  // residual is not below ||b||
+  void learned_guess(double bnorm, fs_stats& s) {
+    hipEvent_t a, e;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&e), "event");
+    hip_check(hipEventRecord(a, st), "event record");
+    const float* Y = fs::guess_infer(model, guess_nc, g_logeta.as<float>(), st);
+    GuessNorm gn;
+    for (int c = 0; c < 4; ++c) { gn.mean[c] = model->mean[c]; gn.stdv[c] = model->stdv[c]; }
+    gn.P = model->P;
+    const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL(k_guess_scatter, dim3(g), dim3(BLOCK), 0, st, n, g_idx.as<int32_t>(), Y, gn,
+                       dptr<double>(x));
+    const int64_t nd = (int64_t)diag_only.size();
+    if (nd > 0)
+      hipLaunchKernelGGL(k_diag_rows, dim3(std::min<int64_t>(1024, (nd + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, nd,
+                         g_diag_rows.as<int32_t>(), K.rp.as<int64_t>(), K.v64.as<double>(), dptr<double>(b),
+                         dptr<double>(x));
+    kmult64(dptr<double>(x), dptr<double>(r), dptr<double>(b));
+    norm(dptr<double>(r), n, SC_TMP + 3);
+    const double res = read_scalar(SC_TMP + 3);
+    s.guess_rel_residual = res / bnorm;
+    if (res < bnorm) {
+      s.guess_used = FS_GUESS_LEARNED;
+    } else {
+      hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
+      s.guess_used = FS_GUESS_ZERO;
+    }
+    hip_check(hipEventRecord(e, st), "event record");
+    hip_check(hipEventSynchronize(e), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, a, e), "elapsed");
+    s.t_guess_ms = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    cnt.launches += 3;
+  }
+
   // rhs, x: device FP64 in caller order
   int solve_device(const double* d_rhs, double* d_x, fs_stats& s);
 };
@@ -450,6 +515,9 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
   }
   h.K.upload_pattern(Kh, true);
   h.hK.assign(Kh.nnz(), 0.0);
+  h.diag_only.clear();
+  for (int64_t r = 0; r < n; ++r)
+    if (Kh.rp[r + 1] - Kh.rp[r] == 1 && Kh.ci[Kh.rp[r]] == r) h.diag_only.push_back((int32_t)r);
   // A alone (velocity rows, velocity columns) in solver order
   HostCsr Ah;
   Ah.n_rows = Ah.n_cols = nu;
@@ -519,10 +587,12 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
   const int32_t* pm = perm_all.as<int32_t>();
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, st, n, pm, d_rhs, dptr<double>(b), 0);
-  if (o.guess == FS_GUESS_ZERO)
-    hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
-  else
+  if (o.guess == FS_GUESS_LEARNED && !model)
+    throw FsError(FS_ERR_BAD_INPUT, "FS_GUESS_LEARNED needs fs_stokes_set_guess_field first");
+  if (o.guess == FS_GUESS_USER)
     hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, st, n, pm, d_x, dptr<double>(x), 0);
+  else
+    hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
   hip_check(hipMemsetAsync(flags.p, 0, 16, st), "memset");  // flags[1] CG iterations, [2] CG failure
   cnt.launches += 2;
   norm(dptr<double>(b), n, SC_TMP);
@@ -532,7 +602,8 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   const double bnorm = h_pin[0], xnorm = h_pin[1];
   if (!std::isfinite(bnorm) || !std::isfinite(xnorm))
     throw FsError(FS_ERR_BAD_INPUT, "rhs or initial guess contains NaN/Inf");
-  s.guess_used = o.guess == FS_GUESS_ZERO ? FS_GUESS_ZERO : FS_GUESS_USER;
+  s.guess_used = o.guess == FS_GUESS_USER ? FS_GUESS_USER : FS_GUESS_ZERO;
+  if (o.guess == FS_GUESS_LEARNED && bnorm > 0.0) learned_guess(bnorm, s);
   const int max_steps = o.max_outer > 0 ? o.max_outer : (int)n;
   if (bnorm == 0.0) {
     hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
@@ -811,11 +882,55 @@ int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* 
   return FS_OK;
 }
 
-int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model*, const float*, int32_t, const int64_t*,
-                              const int8_t*, const int64_t*) {
-  (void)h;
-  fs::set_error("fs_stokes_set_guess_field: learned guess not available in this build");
-  return FS_ERR_UNSUPPORTED;
+int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const float* log_eta, int32_t n_cells,
+                              const int64_t* vnode, const int8_t* vcomp, const int64_t* pnode) {
+  if (!h || !model || !log_eta || !vnode || !vcomp || !pnode || n_cells < 1) {
+    fs::set_error("fs_stokes_set_guess_field: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  FS_GUARD_BEGIN
+  const int d = model->cfg.dim;
+  int64_t P = 1, Pq = 1;
+  for (int a = 0; a < d; ++a) { P *= 2 * (int64_t)n_cells + 1; Pq *= (int64_t)n_cells + 1; }
+  if ((d + 1) * P >= INT32_MAX) throw FsError(FS_ERR_BAD_INPUT, "lattice too large");
+  for (int64_t i = 0; i < Pq; ++i)
+    if (!std::isfinite(log_eta[i])) throw FsError(FS_ERR_BAD_INPUT, "log10(eta) field has NaN/Inf");
+  std::vector<int32_t> idx(h->n);
+  for (int64_t r = 0; r < h->nu; ++r) {
+    const int64_t c = h->FA.sched.perm[r];
+    const int64_t lat = vnode[c];
+    const int comp = vcomp[c];
+    if (lat < 0 || lat >= P || comp < 0 || comp >= d) throw FsError(FS_ERR_BAD_INPUT, "velocity dof map out of range");
+    idx[r] = (int32_t)(comp * P + lat);
+  }
+  for (int64_t q = 0; q < h->np; ++q) {
+    const int64_t c = h->FS.sched.perm[q];
+    int64_t t = pnode[c];
+    if (t < 0 || t >= Pq) throw FsError(FS_ERR_BAD_INPUT, "pressure dof map out of range");
+    int64_t lat = 0, stride = 1;
+    for (int a = 0; a < d; ++a) {
+      lat += 2 * (t % (n_cells + 1)) * stride;
+      t /= (n_cells + 1);
+      stride *= 2 * (int64_t)n_cells + 1;
+    }
+    idx[h->nu + q] = (int32_t)(d * P + lat);
+  }
+  h->g_idx.upload(idx.data(), 4 * idx.size());
+  h->g_logeta.upload(log_eta, 4 * (size_t)Pq);
+  if (!h->diag_only.empty()) h->g_diag_rows.upload(h->diag_only.data(), 4 * h->diag_only.size());
+  h->model = model;
+  h->guess_nc = n_cells;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess) {
+  if (!h || guess < FS_GUESS_USER || guess > FS_GUESS_LEARNED) {
+    fs::set_error("fs_stokes_set_guess_mode: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  h->o.guess = guess;
+  return FS_OK;
 }
 
 int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t reps, double* ms_per_launch,

@@ -161,6 +161,24 @@ class StokesLinearSystem:
         self.last_stats = st.as_dict()
         return rc, self.last_stats
 
+    def set_guess_field(self, model, system):
+        """FS_GUESS_LEARNED input: the network `model` (felspa_amd.guess.GuessModel,
+        kept alive here) and the system's log10(eta) field and dof lattice maps."""
+        le = np.ascontiguousarray(system.log_eta, dtype=np.float32)
+        vn = np.ascontiguousarray(system.vnode, dtype=np.int64)
+        vc = np.ascontiguousarray(system.vcomp, dtype=np.int8)
+        pn = np.ascontiguousarray(system.pnode, dtype=np.int64)
+        L.check(self._lib.fs_stokes_set_guess_field(self.h, model.h, le.ctypes.data_as(L.P(L.f32)),
+                                                    int(system.n_cells), vn.ctypes.data_as(L.P(L.i64)),
+                                                    vc.ctypes.data_as(L.P(C.c_int8)),
+                                                    pn.ctypes.data_as(L.P(L.i64))),
+                "fs_stokes_set_guess_field")
+        self._guess_model = model
+
+    def set_guess_mode(self, guess):
+        L.check(self._lib.fs_stokes_set_guess_mode(self.h, int(guess)), "fs_stokes_set_guess_mode")
+        self.opts.guess = guess
+
     def vmult(self, x, precision=L.FS_PREC_FP64):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n)

@@ -99,6 +99,9 @@ typedef struct fs_stats {
   double t_total_ms;
   double bytes_algorithmic;  /* sum over kernels of their algorithmic HBM bytes */
   int64_t n_kernel_launches;
+  double t_guess_ms;         /* learned guess: network inference + scatter + residual check
+                                (part of t_krylov_ms) */
+  double guess_rel_residual; /* ||b - K x0|| / ||b|| of the learned guess before the fallback test */
 } fs_stats;
 
 typedef struct fs_handle fs_handle;
@@ -133,14 +136,23 @@ int fs_stokes_vmult(fs_handle* h, const double* src, double* dst, int32_t precis
 int fs_stokes_precond_vmult(fs_handle* h, const double* src, double* dst, int32_t precision,
                             fs_stats* stats);
 
-/* Set the warm-start field for FS_GUESS_LEARNED: log10(eta) on the pressure
- * node lattice, (N+1)^dim values, x fastest, plus the lattice maps of the dofs
- * (see fs_system_dof_maps in fs_assemble.h). The model comes from
- * fs_guess_model_* in fs_guess.h. */
+/* Initial-guess slot of StokesLinearSystem::solve_gmres (stokes_ilu.cc:523-524)
+ * for FS_GUESS_LEARNED: the network `model` (fs_guess.h, not owned; it must
+ * outlive the handle's solves) maps log10(eta) on the pressure-node lattice
+ * ((N+1)^dim values, x fastest, host array) to (u, p) on the node lattices;
+ * vnode/vcomp [n_u] and pnode [n_p] place every dof of the caller's order on
+ * them (fs_system_dof_maps in fs_assemble.h). In each solve the guess is the
+ * network output at the dofs, with every row that holds only its diagonal
+ * (constrained rows) set to rhs/diagonal; if its residual is not below
+ * ||rhs|| the solve starts from zero instead (fs_stats.guess_used says which). */
 struct fs_guess_model;
 int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const float* log_eta,
                               int32_t n_cells, const int64_t* vnode, const int8_t* vcomp,
                               const int64_t* pnode);
+
+/* Switch the guess mode (fs_opts.guess) of an existing handle between solves,
+ * e.g. to compare FS_GUESS_LEARNED with FS_GUESS_ZERO on the same matrices. */
+int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess);
 
 /* Row permutation used by FS_ORDER_BLOCK_COLOR for the A (which=0) and
  * M_p (which=1) blocks: perm[new] = old. */
