@@ -37,6 +37,73 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def learned_leg(args, log):
+    """Held-out fields (samples 512+) of the 2D ellipse family at 128^2: one
+    mixed-precision handle (numeric refactor per field) timed from the zero
+    guess and from the network's guess (inference inside the solve), and the
+    plain FP64 solve of the same method; inputs resident in HBM."""
+    import torch
+    import felspa_amd as fa
+    from felspa_amd import _lib as L
+    from felspa_amd import guess as G
+    if not os.path.exists(G.DEFAULT_2D):
+        log("learned leg skipped: %s not present" % G.DEFAULT_2D)
+        return None
+    model = G.GuessModel.load(G.DEFAULT_2D)
+    st_cfg = fa.solver_settings("C4")
+    rows = []
+    mixed = plain = None
+    dev = torch.device("cuda:0")
+    for s in range(512, 512 + args.learned_samples):
+        S = fa.StokesSystem(fa.make_problem("C4", sample=s))
+        common = dict(A_lof=st_cfg["ilu_k_A"], S_lof=st_cfg["ilu_k_S"], block_hint=S.block_hint(5),
+                      constraints=S.constrained, restart=args.restart, inner_a_iters=args.inner_a,
+                      max_outer=3000)
+        if mixed is None:
+            mixed = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO, **common)
+            plain = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO,
+                                          inner_cg_tol=1e-2, inner_cg_max=50, **common)
+        else:
+            mixed.refactor(S.blocks)
+            plain.refactor(S.blocks)
+        mixed.set_guess_field(model, S)
+        d_rhs = torch.from_numpy(S.rhs).to(dev)
+        d_x = torch.zeros(S.n, dtype=torch.float64, device=dev)
+        res = {}
+        for label, lin, mode in (("zero", mixed, L.FS_GUESS_ZERO), ("learned", mixed, L.FS_GUESS_LEARNED),
+                                 ("plain_fp64", plain, L.FS_GUESS_ZERO)):
+            lin.set_guess_mode(mode)
+            lin.solve_device(d_rhs.data_ptr(), d_x.data_ptr())  # warm-up (numeric factorisation cached)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rc, st = lin.solve_device(d_rhs.data_ptr(), d_x.data_ptr())
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) * 1e3
+            if rc != 0:
+                raise RuntimeError("learned leg: %s solve did not converge" % label)
+            res[label] = dict(ms=round(ms, 3), outer=st["outer_iters"], res=st["true_rel_residual"],
+                              guess_used=st["guess_used"], guess_rel_residual=st["guess_rel_residual"],
+                              t_guess_ms=round(st["t_guess_ms"], 3))
+        log("learned leg sample %d: zero %.1f ms (%d outer), learned %.1f ms (%d outer, guess res %.2e), "
+            "plain fp64 %.1f ms (%d outer)" % (s, res["zero"]["ms"], res["zero"]["outer"], res["learned"]["ms"],
+                                               res["learned"]["outer"], res["learned"]["guess_rel_residual"],
+                                               res["plain_fp64"]["ms"], res["plain_fp64"]["outer"]))
+        rows.append(dict(sample=s, **{k: v for k, v in res.items()}))
+    mean = lambda k, f: float(np.mean([r[k][f] for r in rows]))
+    out = {"workload": "C4: " + WORKLOADS["C4"] + "; held-out samples 512..%d" % (511 + args.learned_samples),
+           "network": dict(model.cfg, n_params=model.n_params, weights=os.path.relpath(G.DEFAULT_2D, ROOT)),
+           "ms_mixed_learned": round(mean("learned", "ms"), 3), "ms_mixed_zero": round(mean("zero", "ms"), 3),
+           "ms_plain_fp64": round(mean("plain_fp64", "ms"), 3),
+           "outer_mixed_learned": mean("learned", "outer"), "outer_mixed_zero": mean("zero", "outer"),
+           "outer_plain_fp64": mean("plain_fp64", "outer"),
+           "guess_rel_residual": mean("learned", "guess_rel_residual"),
+           "t_guess_ms": round(mean("learned", "t_guess_ms"), 3)}
+    out["speedup_mixed_learned_vs_plain_fp64"] = round(out["ms_plain_fp64"] / out["ms_mixed_learned"], 3)
+    out["speedup_learned_vs_zero_guess"] = round(out["ms_mixed_zero"] / out["ms_mixed_learned"], 3)
+    out["samples"] = rows
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,12 +112,14 @@ def main():
     ap.add_argument("--config", default="C2")
     ap.add_argument("--n-cells", type=int, default=None)
     ap.add_argument("--inner-a", type=int, default=60)
-    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--restart", type=int, default=60)
     ap.add_argument("--hint", type=int, default=None, help="lattice nodes per block edge (default 5 in 2D, 3 in 3D)")
     ap.add_argument("--ilu-k", type=int, default=None, help="override the configuration's ILU level on A")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start leg (C4 fields)")
+    ap.add_argument("--learned-samples", type=int, default=4, help="held-out C4 fields in the learned leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the solve is not sharded across GPUs "
@@ -74,13 +143,15 @@ def main():
         args.hint = 5 if S.dim == 2 else 3
     hint = S.block_hint(args.hint)
     common = dict(A_lof=kA, S_lof=kS, block_hint=hint, constraints=S.constrained,
-                  restart=args.restart, inner_a_iters=args.inner_a)
+                  restart=args.restart, inner_a_iters=args.inner_a, max_outer=3000)
     t0 = time.time()
     mixed = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO, **common)
     log("mixed handle: setup (orderings + symbolic ILU + upload) %.1fs" % (time.time() - t0))
     plain = None
     if not args.no_plain:
-        plain = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO, **common)
+        # the same method entirely in FP64 (same inner tolerances and counts)
+        plain = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO,
+                                      inner_cg_tol=1e-2, inner_cg_max=50, **common)
 
     dev = torch.device("cuda:0")
     d_rhs = torch.from_numpy(S.rhs).to(dev)
@@ -136,25 +207,39 @@ def main():
         e = errs(x_plain)
         out_extra.update(plain_err_u_vs_tight=e[0], plain_err_p_vs_tight=e[1])
 
-    # roofline: dominant kernel = the inner-PCG A-block SpMV (FP32) unless the
-    # ILU application costs more per solve; both are timed with HIP events on
-    # the solver's stream
-    kinds = {0: "A-block SpMV (k_spmv_dot, FP32)", 2: "ILU(k) apply on A (k_ilu_stage, FP32)",
-             1: "K SpMV (k_spmv, FP64)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
+    # roofline of the dominant kernel class of the mixed solve, timed with HIP
+    # events on the solver's own stream. Per mixed solve every inner PCG step
+    # runs one A-block SpMV (kind 0) and one ILU(k) application (kind 2, all
+    # its stage launches); the class with the larger share of the solve is
+    # reported, its unit being one SpMV launch or one ILU application.
+    _, n_stages = mixed.ordering(0)
+    kinds = {0: "A-block SpMV (k_spmv4_dot, FP32)",
+             2: "ILU(k) apply on A (k_ilu_stage_dense fwd+bwd, FP32)",
+             1: "K SpMV (k_spmv4, FP64)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
+    launches = {0: 1, 2: 2 * n_stages, 1: 1, 3: 1}
     timing = {}
     for kind in (0, 2, 1, 3):
         ms, by = mixed.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED, reps=30)
         timing[kind] = (ms, by)
-        log("kernel %-40s %.4f ms/launch  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
+        log("kernel %-52s %.4f ms/unit  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
     n_inner = st_mixed["inner_iters"]
-    dom = 0
+    pcg_steps = st_mixed["outer_iters"] * args.inner_a
+    share = {k: pcg_steps * timing[k][0] / st_mixed["t_total_ms"] for k in (0, 2)}
+    dom = 2 if share[2] >= share[0] else 0
     ms_dom, by_dom = timing[dom]
     achieved = by_dom / (ms_dom * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kinds[dom],
-                "bytes_per_launch": by_dom, "ms_per_launch": round(ms_dom, 5)}
+                "bytes_per_unit": by_dom, "ms_per_unit": round(ms_dom, 5), "launches_per_unit": launches[dom],
+                "share_of_solve": round(share[dom], 3)}
     out_extra["kernel_gbs"] = {kinds[k]: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in timing.items()}
+    out_extra["kernel_ms_per_unit"] = {kinds[k]: round(v[0], 5) for k, v in timing.items()}
     out_extra["solve_gbs_algorithmic"] = round(st_mixed["bytes_algorithmic"] / (st_mixed["t_total_ms"] * 1e-3) / 1e9, 1)
+
+    # learned warm start: held-out fields of the ellipse family (2D 128^2),
+    # mixed + learned guess vs mixed + zero guess vs the plain FP64 solve
+    if not args.no_learned:
+        out_extra["learned"] = learned_leg(args, log)
 
     # CPU baseline: the oracle's FP64 restatement of the same algorithm on the
     # host cores, on a bounded sample (a few outer iterations), scaled to one
@@ -167,7 +252,7 @@ def main():
         outer = args.cpu_outer
         t = time.perf_counter()
         _, cst = oracle_py.stokes_solve(S.blocks, S.rhs, kA=kA, kS=kS, restart=args.restart, rtol=1e-8,
-                                        method=2, threads=threads, max_steps=outer,
+                                        method=2, threads=threads, max_steps=outer, inner_tol=1e-2,
                                         inner_a_iters=args.inner_a)
         wall = time.perf_counter() - t
         per_outer = cst["t_krylov_s"] / max(1, cst["outer_iters"])

@@ -11,7 +11,7 @@ namespace fsk {
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;      // threads per workgroup for streaming kernels
 constexpr int RED_BLOCKS = 512; // partial-sum slots of the two-stage reductions
-constexpr int MAXM = 32;        // max vectors in one multi-dot (restart + 1)
+constexpr int MAXM = 64;        // max vectors in one multi-dot (restart + 1)
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -170,6 +170,31 @@ __device__ __forceinline__ TACC row_dot4(const i32x4* __restrict__ ci4, const ty
   return group_reduce<G>(acc);
 }
 
+// Same product with two groups of 4 per lane in flight: both groups' column
+// and value loads are issued before any gather (the second group is clamped
+// to the first when the row has no more, and masked), so a lane pays one
+// load latency and one gather latency per two groups.
+template <typename TV, typename TX, typename TACC, int G>
+__device__ __forceinline__ TACC row_dot4x2(const i32x4* __restrict__ ci4,
+                                           const typename Vec4<TV>::type* __restrict__ val4,
+                                           const TX* __restrict__ x, int64_t q0, int64_t q1, int lane) {
+  TACC acc = 0, acc2 = 0;
+  for (int64_t q = q0 + lane; q < q1; q += 2 * G) {
+    const bool two = q + G < q1;
+    const int64_t qb = two ? q + G : q;
+    const i32x4 c = __builtin_nontemporal_load(ci4 + q);
+    const typename Vec4<TV>::type v = __builtin_nontemporal_load(val4 + q);
+    const i32x4 c2 = __builtin_nontemporal_load(ci4 + qb);
+    const typename Vec4<TV>::type v2 = __builtin_nontemporal_load(val4 + qb);
+    const TX x0 = x[c.x], x1 = x[c.y], x2 = x[c.z], x3 = x[c.w];
+    const TX y0 = x[c2.x], y1 = x[c2.y], y2 = x[c2.z], y3 = x[c2.w];
+    acc += (TACC)v.x * (TACC)x0 + (TACC)v.y * (TACC)x1 + (TACC)v.z * (TACC)x2 + (TACC)v.w * (TACC)x3;
+    const TACC s2 = (TACC)v2.x * (TACC)y0 + (TACC)v2.y * (TACC)y1 + (TACC)v2.z * (TACC)y2 + (TACC)v2.w * (TACC)y3;
+    acc2 += two ? s2 : (TACC)0;
+  }
+  return group_reduce<G>(acc + acc2);
+}
+
 // y = M x / b - M x / M x - b on a CSR4 matrix (rp4 counts groups of 4)
 template <typename TV, typename TX, typename TY, typename TB, typename TACC, int G, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off, const int64_t* __restrict__ rp4,
@@ -182,7 +207,8 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off,
   const int lane = (int)(gid % G);
   if (row >= nrows) return;
   const int64_t r = row + row_off;
-  const TACC acc = row_dot4<TV, TX, TACC, G, MODE == 2>(ci4, val4, x, rp4[r], rp4[r + 1], lane, col_limit);
+  const TACC acc = MODE == 2 ? row_dot4<TV, TX, TACC, G, true>(ci4, val4, x, rp4[r], rp4[r + 1], lane, col_limit)
+                             : row_dot4x2<TV, TX, TACC, G>(ci4, val4, x, rp4[r], rp4[r + 1], lane);
   if (lane == 0) {
     if (MODE == 0) y[row] = (TY)acc;
     else if (MODE == 1) y[row] = (TY)((TACC)b[row] - acc);
@@ -202,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
   const int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G;
   double contrib = 0;
   if (row < nrows) {
-    const T acc = row_dot4<T, T, T, G, false>(ci4, val4, p, rp4[row], rp4[row + 1], lane, 0);
+    const T acc = row_dot4x2<T, T, T, G>(ci4, val4, p, rp4[row], rp4[row + 1], lane);
     if (lane == 0) {
       q[row] = acc;
       contrib = (double)acc * (double)p[row];
@@ -410,6 +436,18 @@ static __global__ __launch_bounds__(BLOCK) void k_gather_vals(int64_t n, const i
   }
 }
 
+// d64[i] = src[pos[i]] (0 where pos[i] < 0: padding), d32 = (float) d64
+static __global__ __launch_bounds__(BLOCK) void k_gather_vals_pad(int64_t n, const int64_t* __restrict__ pos,
+                                                                  const double* __restrict__ src,
+                                                                  double* __restrict__ d64, float* __restrict__ d32) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const int64_t p = pos[i];
+    const double v = p >= 0 ? src[p] : 0.0;
+    d64[i] = v;
+    d32[i] = (float)v;
+  }
+}
+
 // dst[map[e]] = src[e]
 static __global__ __launch_bounds__(BLOCK) void k_scatter_vals(int64_t n, const int64_t* __restrict__ map,
                                                         const double* __restrict__ src,
@@ -608,6 +646,170 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_stage(IluView<T> F, int blk0, int
     if (FWD) ilu_fwd_block<T, TB, TY, G, UNIT, BLOCK>(F, blk, b, y, L, cap, threadIdx.x);
     else ilu_bwd_block<T, TB, TY, G, UNIT, BLOCK>(F, blk, b, y, L, cap, threadIdx.x);
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with explicit inverses of the in-block factors.
+// For block b (rows r0 .. r0+nr-1) the stage computes
+//   forward   y_b = L_bb^{-1} (b_b - L_b,ext y_ext)
+//   backward  x_b = U_bb^{-1} (y_b - U_b,ext x_ext)
+// which is the substitution of ilu.implement.h:568-594 regrouped by blocks:
+// the rows of one block depend on each other only through L_bb / U_bb, whose
+// inverses (dense triangles, packed by columns) are formed once per numeric
+// factorisation. The sequential chain through a block becomes a dense
+// mat-vec whose column reads are coalesced across the lanes (rows).
+// Packing (nr rows): L^{-1}(i,j), i >= j, at j*nr - j(j-1)/2 + (i-j);
+//                    U^{-1}(i,j), i <= j, at j(j+1)/2 + i.
+__host__ __device__ inline int64_t packed_tri_size(int64_t nr) { return nr * (nr + 1) / 2; }
+
+// Per-block operands of one triangle (L or U) for the dense stage kernel:
+// the block's external entries (columns in other blocks) stored block after
+// block, row by row (xptr[r] .. xptr[r+1] for row r, so a block's entries
+// are one contiguous span), and the packed inverse of its in-block triangle.
+template <typename T>
+struct DenseTri {
+  const int32_t* blk_ptr;
+  const int64_t* xptr;
+  const int32_t* xci;
+  const T* xv;
+  const int64_t* inv_off;
+  const T* inv;
+};
+
+__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int ext_cap, int tsz) {
+  const size_t b = (size_t)tsz * (inv_cap + ext_cap + 256) + 4 * (size_t)ext_cap;
+  return (b + 15) & ~(size_t)15;
+}
+
+// 16-byte vector of T
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+
+// One stage of the forward (FWD) or backward solve, one workgroup per block.
+// Packed triangles start at multiples of 4 elements and every block's span
+// of external entries starts at a multiple of 4 entries (padding: value 0,
+// column of a row of the block), so the block's operands are staged into LDS
+// with 16-byte loads, all of them independent:
+//  1. stage the packed inverse and the external entries (columns, values);
+//  2. G threads per row: t_i = b_i - sum over the row's external entries of
+//     value * finished value of an earlier (forward) / later (backward) stage;
+//  3. G threads per row: y_i = sum_j inv(i, j) t_j from LDS.
+// Blocks with more than ext_cap external entries read them from global memory.
+template <typename T, typename TB, typename TY, int G, bool FWD>
+__global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<T> F, int blk0, int inv_cap, int ext_cap,
+                                                           const TB* b, TY* y) {
+  extern __shared__ __attribute__((aligned(16))) char dsm[];
+  using V = typename Vec16<T>::type;
+  constexpr int VN = Vec16<T>::n;
+  T* Ms = (T*)dsm;
+  T* ev = Ms + inv_cap;
+  T* t = ev + ext_cap;
+  int32_t* ec = (int32_t*)(t + 256);
+  const int blk = blk0 + blockIdx.x;
+  const int tid = threadIdx.x;
+  const int r0 = F.blk_ptr[blk], nr = F.blk_ptr[blk + 1] - r0;
+  const T* M = F.inv + F.inv_off[blk];
+  const int msz = nr * (nr + 1) / 2;
+  const int64_t e0 = F.xptr[r0];
+  const int ne = (int)(F.xptr[r0 + nr] - e0);  // a multiple of 4
+  const bool staged = ne <= ext_cap;
+  {
+    const int nv = (msz + VN - 1) / VN;
+    for (int k = tid; k < nv; k += BLOCK) ((V*)Ms)[k] = __builtin_nontemporal_load((const V*)M + k);
+    if (staged) {
+      for (int k = tid; k < ne / VN; k += BLOCK) ((V*)ev)[k] = __builtin_nontemporal_load((const V*)(F.xv + e0) + k);
+      for (int k = tid; k < ne / 4; k += BLOCK)
+        ((i32x4*)ec)[k] = __builtin_nontemporal_load((const i32x4*)(F.xci + e0) + k);
+    }
+  }
+  __syncthreads();
+  const int grp = tid / G, gl = tid % G;
+  for (int i = grp; i < nr; i += BLOCK / G) {
+    const int r = r0 + i;
+    const int p0 = (int)(F.xptr[r] - e0), p1 = (int)(F.xptr[r + 1] - e0);
+    T acc = 0;
+    for (int p = p0 + gl; p < p1; p += 4 * G) {
+      T v[4];
+      TY xg[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q0 = p + u * G;
+        const int q = q0 < p1 ? q0 : p1 - 1;
+        const int32_t c = staged ? ec[q] : F.xci[e0 + q];
+        v[u] = q0 < p1 ? (staged ? ev[q] : F.xv[e0 + q]) : (T)0;
+        xg[u] = y[c];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u] * (T)xg[u];
+    }
+    acc = group_reduce<G>(acc);
+    if (gl == 0) t[i] = (T)b[r] - acc;
+  }
+  __syncthreads();
+  for (int i = grp; i < nr; i += BLOCK / G) {
+    T acc = 0;
+    if (FWD) {  // columns j <= i; column j starts at j nr - j (j - 1) / 2
+      for (int j = gl; j <= i; j += G) acc += Ms[j * nr - j * (j - 1) / 2 + i - j] * t[j];
+    } else {    // columns j >= i; column j starts at j (j + 1) / 2
+      for (int j = i + gl; j < nr; j += G) acc += Ms[j * (j + 1) / 2 + i] * t[j];
+    }
+    acc = group_reduce<G>(acc);
+    if (gl == 0) y[r0 + i] = (TY)acc;
+  }
+}
+
+// Inverses of the in-block factors of every block, FP64: thread j of the
+// block's workgroup forms column j of L_bb^{-1} (unit lower) by forward
+// substitution and column j of U_bb^{-1} by backward substitution, reading
+// the in-block entries of the factor rows and its own finished column.
+static __global__ __launch_bounds__(BLOCK) void k_ilu_block_inverse(IluView<double> F, int nblk,
+                                                                     const int64_t* __restrict__ inv_off,
+                                                                     double* __restrict__ linv,
+                                                                     double* __restrict__ uinv) {
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int r0 = F.blk_ptr[blk], nr = F.blk_ptr[blk + 1] - r0;
+    double* Lb = linv + inv_off[blk];
+    double* Ub = uinv + inv_off[blk];
+    for (int j = threadIdx.x; j < nr; j += blockDim.x) {
+      double* xl = Lb + (int64_t)j * nr - (int64_t)j * (j - 1) / 2;  // rows j..nr-1
+      xl[0] = 1.0;
+      for (int i = j + 1; i < nr; ++i) {
+        const int r = r0 + i;
+        const int64_t base = F.rp[r];
+        double s = 0;
+        for (int p = F.o_lint[r]; p < F.o_diag[r]; ++p) {
+          const int k = F.ci[base + p] - r0;
+          if (k >= j) s += F.v[base + p] * xl[k - j];
+        }
+        xl[i - j] = -s;
+      }
+      double* xu = Ub + (int64_t)j * (j + 1) / 2;  // rows 0..j
+      {
+        const int r = r0 + j;
+        xu[j] = 1.0 / F.v[F.rp[r] + F.o_diag[r]];
+      }
+      for (int i = j - 1; i >= 0; --i) {
+        const int r = r0 + i;
+        const int64_t base = F.rp[r];
+        double s = 0;
+        for (int p = F.o_diag[r] + 1; p < F.o_uext[r]; ++p) {
+          const int k = F.ci[base + p] - r0;
+          if (k <= j) s += F.v[base + p] * xu[k];
+        }
+        xu[i] = -s / F.v[base + F.o_diag[r]];
+      }
+    }
   }
 }
 

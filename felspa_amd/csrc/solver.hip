@@ -274,6 +274,81 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   uv64.alloc(8 * upo.size());
   lv32.alloc(4 * lpo.size());
   uv32.alloc(4 * upo.size());
+  // packed in-block inverses and block-contiguous external entries for the
+  // dense stage kernel (blocks of at most 256 rows: one block's right-hand
+  // side sits in LDS)
+  dense = dense && sched.max_block_rows <= 256;
+  if (dense) {
+    // packed triangles, each starting at a multiple of 4 elements
+    std::vector<int64_t> off(sched.n_blocks() + 1, 0);
+    int64_t cap_inv = 0;
+    for (int b = 0; b < sched.n_blocks(); ++b) {
+      const int64_t sz = (packed_tri_size(sched.blk_ptr[b + 1] - sched.blk_ptr[b]) + 3) & ~int64_t(3);
+      off[b + 1] = off[b] + sz;
+      cap_inv = std::max(cap_inv, sz);
+    }
+    inv_total = off.back();
+    inv_cap = (int)cap_inv;
+    inv_off.upload(off.data(), 8 * off.size());
+    linv64.alloc(8 * std::max<int64_t>(4, inv_total));
+    uinv64.alloc(8 * std::max<int64_t>(4, inv_total));
+    linv32.alloc(4 * std::max<int64_t>(4, inv_total));
+    uinv32.alloc(4 * std::max<int64_t>(4, inv_total));
+    // external entries, L part [0, o_lint) and U part [o_uext, len) of every
+    // row, block after block; every block's span padded to a multiple of 4
+    // (padding: position -1 = value 0, column = the block's first row)
+    std::vector<int64_t> xl(n + 1, 0), xu(n + 1, 0);
+    int64_t ql = 0, qu = 0;
+    xl_max = xu_max = 0;
+    for (int b = 0; b < sched.n_blocks(); ++b) {
+      const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+      const int64_t sl = ql, su = qu;
+      for (int32_t r = r0; r < r1; ++r) {
+        xl[r] = ql;
+        xu[r] = qu;
+        ql += ol[r];
+        qu += F.rp[r + 1] - F.rp[r] - ou[r];
+      }
+      ql = (ql + 3) & ~int64_t(3);
+      qu = (qu + 3) & ~int64_t(3);
+      xl_max = std::max<int>(xl_max, (int)(ql - sl));
+      xu_max = std::max<int>(xu_max, (int)(qu - su));
+    }
+    xl[n] = ql;
+    xu[n] = qu;
+    n_xl = ql;
+    n_xu = qu;
+    std::vector<int32_t> lci(std::max<int64_t>(4, n_xl)), uci(std::max<int64_t>(4, n_xu));
+    std::vector<int64_t> lpo2(std::max<int64_t>(4, n_xl), -1), upo2(std::max<int64_t>(4, n_xu), -1);
+    for (int b = 0; b < sched.n_blocks(); ++b) {
+      const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+      for (int64_t q = xl[r0]; q < xl[r1]; ++q) lci[q] = r0;
+      for (int64_t q = xu[r0]; q < xu[r1]; ++q) uci[q] = r0;
+      for (int32_t r = r0; r < r1; ++r) {
+        const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
+        int64_t q = xl[r];
+        for (int32_t p = 0; p < ol[r]; ++p, ++q) {
+          lci[q] = F.ci[base + p];
+          lpo2[q] = base + p;
+        }
+        q = xu[r];
+        for (int64_t p = ou[r]; p < len; ++p, ++q) {
+          uci[q] = F.ci[base + p];
+          upo2[q] = base + p;
+        }
+      }
+    }
+    xlptr.upload(xl.data(), 8 * xl.size());
+    xuptr.upload(xu.data(), 8 * xu.size());
+    xlci.upload(lci.data(), 4 * lci.size());
+    xuci.upload(uci.data(), 4 * uci.size());
+    xlpos.upload(lpo2.data(), 8 * lpo2.size());
+    xupos.upload(upo2.data(), 8 * upo2.size());
+    xlv64.alloc(8 * lci.size());
+    xlv32.alloc(4 * lci.size());
+    xuv64.alloc(8 * uci.size());
+    xuv32.alloc(4 * uci.size());
+  }
   host_pattern = std::move(F);
 }
 
@@ -323,6 +398,23 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
                      (const double*)v64.p, (float*)v32.p);
   cnt.launches++;
   gather_inblock(st);
+  if (dense) {
+    if (n_xl > 0)
+      hipLaunchKernelGGL(k_gather_vals_pad, dim3(grid_for(n_xl)), dim3(BLOCK), 0, st, n_xl, xlpos.as<int64_t>(),
+                         v64.as<double>(), xlv64.as<double>(), xlv32.as<float>());
+    if (n_xu > 0)
+      hipLaunchKernelGGL(k_gather_vals_pad, dim3(grid_for(n_xu)), dim3(BLOCK), 0, st, n_xu, xupos.as<int64_t>(),
+                         v64.as<double>(), xuv64.as<double>(), xuv32.as<float>());
+    const int nb = sched.n_blocks();
+    hipLaunchKernelGGL(k_ilu_block_inverse, dim3(std::max(1, std::min(nb, 65535))), dim3(BLOCK), 0, st, V, nb,
+                       inv_off.as<int64_t>(), linv64.as<double>(), uinv64.as<double>());
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(inv_total)), dim3(BLOCK), 0, st, inv_total,
+                       linv64.as<double>(), linv32.as<float>());
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(inv_total)), dim3(BLOCK), 0, st, inv_total,
+                       uinv64.as<double>(), uinv32.as<float>());
+    cnt.launches += 3;
+    cnt.bytes += (double)nnz * 12.0 + 2.0 * inv_total * 12.0;
+  }
 }
 
 // in-block entries in column order, both precisions
@@ -358,10 +450,72 @@ static void launch_stage_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, 
   }
 }
 
+// LDS available to one workgroup of the dense stage kernel (4 waves)
+static size_t dense_wg_lds_limit() {
+  static size_t lim = 0;
+  if (!lim) {
+    int dev = 0, v = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    lim = v > 0 ? (size_t)v : 64 * 1024;
+    lim = std::min<size_t>(lim, 160 * 1024);
+  }
+  return lim;
+}
+
+template <typename T, typename TB, typename TY, int G, bool FWD>
+static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
+  const int nb = b1 - b0;
+  if (nb <= 0) return;
+  const bool d = std::is_same<T, double>::value;
+  DenseTri<T> D;
+  D.blk_ptr = F.blk_ptr.as<int32_t>();
+  D.xptr = FWD ? F.xlptr.as<int64_t>() : F.xuptr.as<int64_t>();
+  D.xci = FWD ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
+  D.xv = FWD ? (d ? (const T*)F.xlv64.p : (const T*)F.xlv32.p) : (d ? (const T*)F.xuv64.p : (const T*)F.xuv32.p);
+  D.inv_off = F.inv_off.as<int64_t>();
+  D.inv = FWD ? (d ? (const T*)F.linv64.p : (const T*)F.linv32.p) : (d ? (const T*)F.uinv64.p : (const T*)F.uinv32.p);
+  // external entries staged in LDS up to what the workgroup's LDS allows
+  const size_t lim = dense_wg_lds_limit();
+  const size_t fixed = dense_lds_bytes(F.inv_cap, 0, sizeof(T));
+  const int want = FWD ? F.xl_max : F.xu_max;
+  int ext_cap = lim > fixed + 64 ? (int)std::min<size_t>(want, (lim - fixed - 16) / (sizeof(T) + 4)) : 0;
+  ext_cap &= ~3;
+  const size_t lds = dense_lds_bytes(F.inv_cap, ext_cap, sizeof(T));
+  auto fn = &k_ilu_stage_dense<T, TB, TY, G, FWD>;
+  if (lds > 64 * 1024)
+    hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+              "hipFuncSetAttribute");
+  hipLaunchKernelGGL(fn, dim3(nb), dim3(BLOCK), lds, st, D, b0, F.inv_cap, ext_cap, b, y);
+}
+
+template <typename T, typename TB, typename TY, bool FWD>
+static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
+  // threads per row: the block's rows fill the 256-thread workgroup
+  const int mr = F.sched.max_block_rows;
+  if (mr <= 32) launch_dense<T, TB, TY, 8, FWD>(F, b0, b1, b, y, st);
+  else if (mr <= 64) launch_dense<T, TB, TY, 4, FWD>(F, b0, b1, b, y, st);
+  else if (mr <= 128) launch_dense<T, TB, TY, 2, FWD>(F, b0, b1, b, y, st);
+  else launch_dense<T, TB, TY, 1, FWD>(F, b0, b1, b, y, st);
+}
+
 // x = (LU)^{-1} b : forward into tmp, backward into x (tmp may alias x)
 template <typename T, typename TB>
 void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
   const int ns = sched.n_stages();
+  // the dense kernels need a block's packed triangle in LDS
+  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(T)) + 64 <= dense_wg_lds_limit()) {
+    for (int s = 0; s < ns; ++s)
+      launch_dense_g<T, TB, T, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
+    for (int s = ns - 1; s >= 0; --s)
+      launch_dense_g<T, T, T, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
+    cnt.launches += 2 * ns;
+    // external entries (value + column), both packed triangles, the row
+    // pointers of the external spans, b read, y written, gathered values
+    const double ext = (double)(n_xl + n_xu);
+    cnt.bytes += ext * (sizeof(T) + 4) + 2.0 * inv_total * sizeof(T) + 16.0 * n + 4.0 * n * sizeof(T);
+    return;
+  }
   for (int s = 0; s < ns; ++s)
     launch_stage_g<T, TB, T, true, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
   for (int s = ns - 1; s >= 0; --s)

@@ -70,6 +70,15 @@ struct IluDev {
   DevBuf lcp, lrow, lpos, lv64, lv32, ucp, urow, upos, uv64, uv32;
   int64_t n_lin = 0, n_uin = 0;
   int max_rows = 1, cap = 0, max_len = 1, G = 16;
+  // explicit inverses of the in-block triangles (packed per block), used by
+  // apply(); `dense` off keeps the in-block substitution kernels
+  bool dense = true;
+  int64_t inv_total = 0;
+  int inv_cap = 0;                 // largest packed triangle of a block
+  int64_t n_xl = 0, n_xu = 0;      // external entries of L / U
+  int xl_max = 0, xu_max = 0;      // largest per-block count of external L / U entries
+  DevBuf inv_off, linv64, uinv64, linv32, uinv32;
+  DevBuf xlptr, xlci, xlpos, xlv64, xlv32, xuptr, xuci, xupos, xuv64, xuv32;
   void gather_inblock(hipStream_t st);
   void build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max);
   template <typename T>

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -69,6 +70,74 @@ T* dptr(const DevBuf& b, int64_t off = 0) {
   return (T*)b.p + off;
 }
 
+// StokesScalingOperator::initialize (stokes_common.implement.h:13-53), rows
+// of K in solver order: D[r] = sqrt(max_c |A_rc|) for the velocity rows;
+// per-workgroup partials of sum 1/D^2 and of sum (max_c |B^T_rc|)^2
+__global__ __launch_bounds__(BLOCK) void k_scale_coeffs_u(int64_t nu, const int64_t* __restrict__ rp,
+                                                          const int32_t* __restrict__ ci,
+                                                          const double* __restrict__ v, double* __restrict__ D,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[BLOCK / WAVE];
+  double s1 = 0, s2 = 0;
+  for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < nu; r += (int64_t)gridDim.x * BLOCK) {
+    double ma = 0, mb = 0;
+    for (int64_t p = rp[r]; p < rp[r + 1]; ++p) {
+      const double a = fabs(v[p]);
+      if (ci[p] < nu) ma = fmax(ma, a);
+      else mb = fmax(mb, a);
+    }
+    D[r] = sqrt(ma);
+    s1 += 1.0 / ma;
+    s2 += mb * mb;
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// pressure coefficient s_p = P sqrt(P g) / n_p, P = sum 1/D_u^2, g = sum ghat^2,
+// written to D[nu .. nu+np) (one workgroup; partials summed in index order)
+__global__ __launch_bounds__(BLOCK) void k_scale_coeffs_p(int nb, const double* __restrict__ part, int64_t nu,
+                                                          int64_t np, double* __restrict__ D) {
+  __shared__ double sh[BLOCK / WAVE];
+  __shared__ double sp;
+  double s1 = 0, s2 = 0;
+  for (int i = threadIdx.x; i < nb; i += BLOCK) {
+    s1 += part[2 * i];
+    s2 += part[2 * i + 1];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) sp = s1 * sqrt(s1 * s2) / (double)np;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < np; i += BLOCK) D[nu + i] = sp;
+}
+
+// y = x * D (inv = 0) or x / D (inv = 1); y may alias x
+template <typename TA, typename TB>
+__global__ __launch_bounds__(BLOCK) void k_mul_diag(int64_t n, const TA* x, const double* __restrict__ D, TB* y,
+                                                    int inv) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    y[i] = (TB)(inv ? (double)x[i] / D[i] : (double)x[i] * D[i]);
+}
+
+// y = bs - y / D
+__global__ __launch_bounds__(BLOCK) void k_resid_scaled(int64_t n, const double* __restrict__ bs,
+                                                        const double* __restrict__ D, double* y) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    y[i] = bs[i] - y[i] / D[i];
+}
+
+// y = a x + b y
+__global__ __launch_bounds__(BLOCK) void k_axpby(int64_t n, double a, const double* __restrict__ x, double b,
+                                                 double* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+    y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
 struct GuessNorm {
   float mean[4], stdv[4];
   int64_t P;  // lattice points per output channel
@@ -111,6 +180,12 @@ struct fs_handle {
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
   Counters cnt;
+  DevBuf scr_ws;  // Schur-complement reduction workspace (FS_METHOD_SCR)
+  // diagonal scaling (StokesSolverControl::apply_diagonal_scaling): while a
+  // scaled solve runs, Dsc points at D (solver order) and the Krylov methods
+  // see K~ = D^-1 K D^-1 and P~^-1 = D P^-1 D on x~ = D x, b~ = D^-1 b
+  const double* Dsc = nullptr;
+  DevBuf dsc_buf;
   // learned guess (FS_GUESS_LEARNED)
   fs_guess_model* model = nullptr;
   int guess_nc = 0;
@@ -171,7 +246,9 @@ struct fs_handle {
     a.q = w + 3 * lp;
     a.partial = dptr<double>(partial);
     const bool mixed = std::is_same<T, float>::value;
-    a.tol = mixed ? 0.0 : o.inner_cg_tol;
+    // InverseMatrixCG's ReductionControl(tol, reduce) for the reference-mode
+    // GMRES and SCR; the flexible method only needs a relative reduction
+    a.tol = (mixed || o.method == FS_METHOD_FGMRES) ? 0.0 : o.inner_cg_tol;
     a.reduce = mixed ? o.inner_cg_tol_mixed : o.inner_cg_tol;
     a.max_it = mixed ? (o.inner_cg_max_mixed > 0 ? o.inner_cg_max_mixed : (int)np)
                      : (o.inner_cg_max > 0 ? o.inner_cg_max : (int)np);
@@ -324,10 +401,12 @@ struct fs_handle {
     bool conv = false;
     double est = 0;
     for (;;) {
-      kmult64(dptr<double>(x), Vb, dptr<double>(b));  // V0 = b - K x
+      kop(dptr<double>(x), Vb, dptr<double>(b));  // V0 = b - K x
       norm(Vb, n, SC_NRM);
       const double beta = read_scalar(SC_NRM);
       est = beta;
+      if (o.verbose) std::fprintf(stderr, "FGMRES restart %d: true residual %.6e (tol %.3e) after %d steps\n", cycles,
+                                  beta, tol_abs, acc);
       if (cycles == 0) s.initial_rel_residual = beta;
       if (!(beta > tol_abs)) { conv = std::isfinite(beta); break; }
       if (acc >= max_steps) break;
@@ -336,14 +415,13 @@ struct fs_handle {
       for (int j = 0; j < m && acc < max_steps; ++j) {
         ++acc;
         T* zj = Zb + (int64_t)j * ld;
-        precond<T>(Vb + (int64_t)j * ld, zj);
+        prec<T>(Vb + (int64_t)j * ld, zj);
         double* w = Vb + (int64_t)(j + 1) * ld;
-        spmv4_launch<T, T, double, double, T, 0>(K, n, 0, (const T*)zj, w, (const double*)nullptr, 0, st);
-        cnt.launches++;
-        cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
+        kop_t<T>(zj, w);
         arnoldi_tail(j);
         dim = j + 1;
         est = read_scalar(SC_EST);
+        if (o.verbose > 1) std::fprintf(stderr, "  FGMRES step %d: estimate %.6e\n", acc, est);
         if (!(est > tol_abs) || !std::isfinite(est)) break;
       }
       finish_cycle<T>(dim, Zb);
@@ -368,8 +446,8 @@ struct fs_handle {
     int acc = 0, cycles = 0;
     Ctrl::State stt = Ctrl::ITERATE;
     do {
-      kmult64(dptr<double>(x), p, dptr<double>(b));
-      precond<double>(p, Vb);
+      kop(dptr<double>(x), p, dptr<double>(b));
+      prec<double>(p, Vb);
       norm(Vb, n, SC_NRM);
       const double rho = read_scalar(SC_NRM);
       stt = c.check(acc, rho);
@@ -378,8 +456,8 @@ struct fs_handle {
       int dim = 0;
       for (int j = 0; j < m && stt == Ctrl::ITERATE; ++j) {
         ++acc;
-        kmult64(Vb + (int64_t)j * ld, p, nullptr);
-        precond<double>(p, Vb + (int64_t)(j + 1) * ld);
+        kop(Vb + (int64_t)j * ld, p, nullptr);
+        prec<double>(p, Vb + (int64_t)(j + 1) * ld);
         arnoldi_tail(j);
         dim = j + 1;
         stt = c.check(acc, read_scalar(SC_EST));
@@ -459,6 +537,235 @@ struct fs_handle {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(e);
     cnt.launches += 3;
+  }
+
+  // ---- diagonal scaling wrappers --------------------------------------------
+  double* dsc_t() { return dptr<double>(dsc_buf, ld); }
+  void* dsc_tT() { return (void*)dptr<double>(dsc_buf, 2 * ld); }
+  void compute_scaling() {
+    if (!dsc_buf.p) dsc_buf.alloc(8 * (size_t)(3 * ld));
+    double* D = dptr<double>(dsc_buf);
+    const int g = std::min<int>(RED_BLOCKS / 2, grid(nu));
+    double* part = dptr<double>(partial);
+    hipLaunchKernelGGL(k_scale_coeffs_u, dim3(g), dim3(BLOCK), 0, st, nu, K.rp.as<int64_t>(), K.ci.as<int32_t>(),
+                       K.v64.as<double>(), D, part);
+    hipLaunchKernelGGL(k_scale_coeffs_p, dim3(1), dim3(BLOCK), 0, st, g, part, nu, np, D);
+    cnt.launches += 2;
+    cnt.bytes += K.nnz * 12.0 * nu / n + 8.0 * n;
+    // s_p is finite and positive only if every velocity coefficient is
+    // (a zero row makes sum 1/D^2 infinite); the reference asserts nonzero
+    double spv = 0;
+    hip_check(hipMemcpyAsync(&spv, D + nu, 8, hipMemcpyDeviceToHost, st), "D2H");
+    sync();
+    if (!(spv > 0.0) || !std::isfinite(spv)) throw FsError(FS_ERR_BAD_INPUT, "diagonal scaling: zero velocity row");
+    Dsc = D;
+  }
+  void scale_vec(double* v, int inv) {
+    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, v, Dsc, v, inv);
+    cnt.launches++;
+  }
+  // y = b~ - K~ x~ (bv given) or y = K~ x~
+  void kop(const double* xv, double* y, const double* bv) {
+    if (!Dsc) { kmult64(xv, y, bv); return; }
+    double* t = dsc_t();
+    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, xv, Dsc, t, 1);
+    kmult64(t, y, nullptr);
+    if (bv) hipLaunchKernelGGL(k_resid_scaled, dim3(grid(n)), dim3(BLOCK), 0, st, n, bv, Dsc, y);
+    else hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, y, Dsc, y, 1);
+    cnt.launches += 2;
+    cnt.bytes += 40.0 * n;
+  }
+  // w = K~ z (z in the preconditioned-vector precision)
+  template <typename T>
+  void kop_t(const T* z, double* w) {
+    if (!Dsc) {
+      spmv4_launch<T, T, double, double, T, 0>(K, n, 0, z, w, (const double*)nullptr, 0, st);
+      cnt.launches++;
+      cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
+      return;
+    }
+    T* t = (T*)dsc_tT();
+    hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, z, Dsc, t, 1);
+    spmv4_launch<T, T, double, double, T, 0>(K, n, 0, t, w, (const double*)nullptr, 0, st);
+    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, w, Dsc, w, 1);
+    cnt.launches += 3;
+    cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0) + n * (2.0 * sizeof(T) + 32.0);
+  }
+  // dst = P~^-1 src
+  template <typename T>
+  void prec(const double* src, T* dst) {
+    if (!Dsc) { precond<T>(src, dst); return; }
+    double* t = dsc_t();
+    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, src, Dsc, t, 0);
+    precond<T>(t, dst);
+    hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, dst, Dsc, dst, 0);
+    cnt.launches += 2;
+    cnt.bytes += n * (24.0 + 2.0 * sizeof(T) + 8.0);
+  }
+
+  // ---- Schur-complement reduction (FS_METHOD_SCR), FP64 -------------------
+  int grid(int64_t len) const { return std::min<int>(4096, (int)std::max<int64_t>(1, (len + BLOCK - 1) / BLOCK)); }
+
+  double dot64(const double* a, const double* b2, int64_t len) {
+    const int g = std::min<int>(RED_BLOCKS, grid(len));
+    hipLaunchKernelGGL((k_dot_partial<double>), dim3(g), dim3(BLOCK), 0, st, len, a, b2, dptr<double>(partial));
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, st, g, dptr<double>(partial), S(SC_TMP + 3), 0);
+    cnt.launches += 2;
+    cnt.bytes += 16.0 * len;
+    return read_scalar(SC_TMP + 3);
+  }
+  void axpby(int64_t len, double a, const double* xv, double bb, double* y) {
+    hipLaunchKernelGGL(k_axpby, dim3(grid(len)), dim3(BLOCK), 0, st, len, a, xv, bb, y);
+    cnt.launches++;
+    cnt.bytes += 24.0 * len;
+  }
+  // y = B^T a (velocity rows of K applied to [0; a]); full: n-vector scratch
+  void bt_mult(const double* a, double* y, double* full) {
+    hip_check(hipMemsetAsync(full, 0, 8 * nu, st), "memset");
+    hip_check(hipMemcpyAsync(full + nu, a, 8 * np, hipMemcpyDeviceToDevice, st), "D2D");
+    spmv4_launch<double, double, double, double, double, 0>(K, nu, 0, full, y, (const double*)nullptr, 0, st);
+    cnt.launches++;
+    cnt.bytes += (double)K.nnz * 12.0 * nu / n;
+  }
+  // y = B v (pressure rows of K applied to [v; 0])
+  void b_mult(const double* v, double* y, double* full) {
+    hip_check(hipMemcpyAsync(full, v, 8 * nu, hipMemcpyDeviceToDevice, st), "D2D");
+    hip_check(hipMemsetAsync(full + nu, 0, 8 * np, st), "memset");
+    spmv4_launch<double, double, double, double, double, 0>(K, np, nu, full, y, (const double*)nullptr, 0, st);
+    cnt.launches++;
+    cnt.bytes += (double)K.nnz * 12.0 * np / n;
+  }
+
+  // InverseMatrixCG::vmult on A (linear_operators.implement.h:64-95): deal.II
+  // SolverCG with ILU_A, ReductionControl(tol, reduce), from x (zero when
+  // x_zero); written in the residual form r = b - A x, z = P r, p = z + beta p,
+  // whose iterates equal deal.II's g = -r, d = -h form.
+  int cg_A(const double* bv, double* xv, bool x_zero, double tol, double reduce, int max_steps, bool& ok) {
+    double* base = (double*)pcg.p;
+    double *rr = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
+    double* part_pq = dptr<double>(pcg_part);
+    double* part_rz = part_pq + spmv_dot_grid();
+    double* rz = S(SC_PCG + 1);
+    const int gv = 512, gu = 256;
+    if (x_zero) {
+      hip_check(hipMemsetAsync(xv, 0, 8 * nu, st), "memset");
+      hipLaunchKernelGGL((k_copy_convert<double, double>), dim3(gv), dim3(BLOCK), 0, st, nu, bv, rr);
+    } else {
+      spmv4_launch<double, double, double, double, double, 1>(Ablk, nu, 0, xv, rr, bv, 0, st);
+    }
+    cnt.launches++;
+    Ctrl c;
+    c.tol = tol;
+    c.reduce = reduce;
+    c.max_steps = max_steps;
+    norm(rr, nu, SC_TMP + 2);
+    Ctrl::State stt = c.check(0, read_scalar(SC_TMP + 2));
+    int it = 0, cur = 0;
+    if (stt == Ctrl::ITERATE) {
+      FA.apply<double, double>(rr, y, z, st, cnt);
+      hipLaunchKernelGGL((k_dot_partial_t<double>), dim3(gv), dim3(BLOCK), 0, st, nu, rr, z, part_rz);
+      hipLaunchKernelGGL((k_pcg_direction<double>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
+      cnt.launches += 2;
+    }
+    while (stt == Ctrl::ITERATE) {
+      ++it;
+      launch_spmv_dot<double>(pv, q, part_pq);
+      hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
+      hipLaunchKernelGGL((k_pcg_update<double>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, xv, rr, 0);
+      cnt.launches += 3;
+      cnt.bytes += Ablk.nnz * 12.0 + 8.0 * (nu + 1) + 8.0 * 7.0 * nu;
+      norm(rr, nu, SC_TMP + 2);
+      const double res = read_scalar(SC_TMP + 2);
+      stt = c.check(it, res);
+      if (stt != Ctrl::ITERATE) break;
+      FA.apply<double, double>(rr, y, z, st, cnt);
+      hipLaunchKernelGGL((k_dot_partial_t<double>), dim3(gv), dim3(BLOCK), 0, st, nu, rr, z, part_rz);
+      hipLaunchKernelGGL((k_pcg_direction<double>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
+                         rz + (cur ^ 1), z, pv, 0);
+      cnt.launches += 2;
+      cur ^= 1;
+    }
+    ok = stt == Ctrl::SUCCESS;
+    return it;
+  }
+
+  // StokesLinearSystem::solve_cg (stokes_ilu.cc:564-710): schur_rhs = B A^{-1}
+  // f - g; CG on S = B A^{-1} B^T preconditioned by InverseMatrixCG(M_p, ILU_S)
+  // to rtol ||schur_rhs|| (ReductionControl with the solver's reduce); then
+  // A u = f - B^T p from the current u. A^{-1} is CG with ILU_A to 1e-15.
+  void scr(fs_stats& s) {
+    const int64_t ldn = (n + 63) & ~int64_t(63), ldv = (nu + 63) & ~int64_t(63), ldp = (np + 63) & ~int64_t(63);
+    if (!scr_ws.p) scr_ws.alloc(8 * (size_t)(ldn + 3 * ldv + 5 * ldp));
+    double* full = dptr<double>(scr_ws);
+    double *tmp = full + ldn, *t2 = tmp + ldv, *t3 = t2 + ldv;
+    double *srhs = t3 + ldv, *g = srhs + ldp, *hh = g + ldp, *d = hh + ldp, *sv = d + ldp;
+    double* xu = dptr<double>(x);
+    double* xp = xu + nu;
+    const double* f = dptr<double>(b);
+    const double* gp = f + nu;
+    int64_t a_steps = 0;
+    bool ok_all = true;
+    auto ainv = [&](const double* rhs_v, double* out, bool from_zero) {
+      bool ok = true;
+      a_steps += cg_A(rhs_v, out, from_zero, 1e-15, 1e-15, (int)nu, ok);
+      ok_all = ok_all && ok;
+    };
+    auto schur = [&](const double* v, double* out) {
+      bt_mult(v, t2, full);
+      ainv(t2, t3, true);
+      b_mult(t3, out, full);
+    };
+    // schur_rhs = B A^{-1} f - g
+    ainv(f, tmp, true);
+    b_mult(tmp, srhs, full);
+    axpby(np, -1.0, gp, 1.0, srhs);
+    Ctrl c;
+    c.tol = o.rtol * std::sqrt(dot64(srhs, srhs, np));
+    c.reduce = o.reduce;
+    c.max_steps = o.max_outer > 0 ? o.max_outer : (int)np;
+    // deal.II SolverCG on S from x_p
+    const double xpn = dot64(xp, xp, np);
+    if (xpn != 0.0) {
+      schur(xp, g);
+      axpby(np, -1.0, srhs, 1.0, g);
+    } else {
+      axpby(np, -1.0, srhs, 0.0, g);
+    }
+    double res = std::sqrt(dot64(g, g, np));
+    Ctrl::State stt = c.check(0, res);
+    int it = 0;
+    double gh = 0;
+    if (stt == Ctrl::ITERATE) {
+      inner_cg<double>(g, hh);
+      axpby(np, -1.0, hh, 0.0, d);
+      gh = dot64(g, hh, np);
+    }
+    while (stt == Ctrl::ITERATE) {
+      ++it;
+      schur(d, hh);
+      const double alpha = gh / dot64(d, hh, np);
+      if (!std::isfinite(alpha)) { stt = Ctrl::FAILURE; break; }
+      axpby(np, alpha, d, 1.0, xp);
+      axpby(np, alpha, hh, 1.0, g);
+      res = std::sqrt(dot64(g, g, np));
+      stt = c.check(it, res);
+      if (stt != Ctrl::ITERATE) break;
+      inner_cg<double>(g, hh);
+      const double ghn = dot64(g, hh, np);
+      const double beta = ghn / gh;
+      gh = ghn;
+      axpby(np, -1.0, hh, beta, d);
+    }
+    (void)sv;
+    // velocity: A u = f - B^T p, from the current u
+    bt_mult(xp, tmp, full);
+    axpby(nu, 1.0, f, -1.0, tmp);
+    ainv(tmp, xu, false);
+    s.outer_iters = it;
+    s.restarts = 0;
+    s.last_value = c.last_value;
+    s.inner_iters = a_steps;  // M_p CG steps are added from the device counter
+    s.status = (stt == Ctrl::SUCCESS && ok_all) ? FS_OK : FS_NOT_CONVERGED;
   }
 
   // rhs, x: device FP64 in caller order
@@ -567,10 +874,10 @@ void fill_stats_err(fs_stats* s, int code) {
 int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   std::memset(&s, 0, sizeof(s));
   cnt = Counters{};
-  if (o.apply_diagonal_scaling)
-    throw FsError(FS_ERR_UNSUPPORTED, "diagonal scaling is not implemented on the GPU path");
-  if (o.precision == FS_PREC_MIXED && o.method == FS_METHOD_GMRES_REF)
-    throw FsError(FS_ERR_UNSUPPORTED, "reference-mode GMRES runs in FP64 only");
+  if (o.apply_diagonal_scaling && o.method == FS_METHOD_SCR)
+    throw FsError(FS_ERR_UNSUPPORTED, "diagonal scaling with SCR is not implemented on the GPU path");
+  if (o.precision == FS_PREC_MIXED && (o.method == FS_METHOD_GMRES_REF || o.method == FS_METHOD_SCR))
+    throw FsError(FS_ERR_UNSUPPORTED, "reference-mode GMRES and SCR run in FP64 only");
   hipEvent_t e0, e1, e2;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
@@ -605,17 +912,35 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   s.guess_used = o.guess == FS_GUESS_USER ? FS_GUESS_USER : FS_GUESS_ZERO;
   if (o.guess == FS_GUESS_LEARNED && bnorm > 0.0) learned_guess(bnorm, s);
   const int max_steps = o.max_outer > 0 ? o.max_outer : (int)n;
+  // pre_solve_scaling (stokes_ilu.cc:418-431): x *= D, rhs /= D; the
+  // tolerance is taken on the scaled rhs (stokes_ilu.cc:533)
+  double bnorm_k = bnorm;
+  Dsc = nullptr;
+  if (o.apply_diagonal_scaling && bnorm != 0.0) {
+    compute_scaling();
+    scale_vec(dptr<double>(b), 1);
+    scale_vec(dptr<double>(x), 0);
+    norm(dptr<double>(b), n, SC_TMP);
+    bnorm_k = read_scalar(SC_TMP);
+  }
   if (bnorm == 0.0) {
     hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
     s.status = FS_OK;
   } else if (o.method == FS_METHOD_GMRES_REF) {
-    gmres_ref(bnorm, max_steps, s);
+    gmres_ref(bnorm_k, max_steps, s);
   } else if (o.method == FS_METHOD_FGMRES) {
-    if (o.precision == FS_PREC_MIXED) fgmres<float>(o.rtol * bnorm, max_steps, s);
-    else fgmres<double>(o.rtol * bnorm, max_steps, s);
-    if (s.initial_rel_residual > 0) s.initial_rel_residual /= bnorm;
+    if (o.precision == FS_PREC_MIXED) fgmres<float>(o.rtol * bnorm_k, max_steps, s);
+    else fgmres<double>(o.rtol * bnorm_k, max_steps, s);
+    if (s.initial_rel_residual > 0) s.initial_rel_residual /= bnorm_k;
+  } else if (o.method == FS_METHOD_SCR) {
+    scr(s);
   } else {
     throw FsError(FS_ERR_UNSUPPORTED, "solution method not implemented");
+  }
+  if (Dsc) {  // post_solve_scaling: x /= D; the rhs back to the caller's values
+    scale_vec(dptr<double>(x), 1);
+    scale_vec(dptr<double>(b), 0);
+    Dsc = nullptr;
   }
   hip_check(hipEventRecord(e2, st), "event record");
   // true residual, back to the caller's order
@@ -635,7 +960,7 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   float ms_setup = 0, ms_kry = 0;
   hip_check(hipEventElapsedTime(&ms_setup, e0, e1), "elapsed");
   hip_check(hipEventElapsedTime(&ms_kry, e1, e2), "elapsed");
-  s.inner_iters = fl[1];
+  s.inner_iters += fl[1];
   s.true_rel_residual = h_pin[2] / (bnorm > 0 ? bnorm : 1.0);
   s.t_setup_ms = ms_setup;
   s.t_krylov_ms = ms_kry;
@@ -653,7 +978,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   if (opts) h.o = *opts;
   else fs_opts_default(&h.o);
   auto& o = h.o;
-  if (o.restart < 1 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [1, 31]");
+  if (o.restart < 1 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [1, 63]");
   if (o.ilu_k_A < 0 || o.ilu_k_S < 0 || o.ilu_k_A > 20 || o.ilu_k_S > 20)
     throw FsError(FS_ERR_BAD_INPUT, "level of fill out of range");
   if (!(o.rtol > 0)) throw FsError(FS_ERR_BAD_INPUT, "rtol must be positive");
@@ -684,6 +1009,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   std::vector<int64_t> mapM;
   HostCsr Mnew = permute_square(hM, sS.perm, sS.iperm, &mapM);
   h.FA.build(pattern_of(Anew), sA, o.ilu_k_A, 8);
+  h.FS.dense = false;  // M_p's factor is applied inside the fused inner CG only
   h.FS.build(pattern_of(Mnew), sS, o.ilu_k_S, 8);
   // K in solver order + value maps from the caller's layout
   build_K(h, A, Bt, B, K11);
@@ -922,6 +1248,15 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const 
   h->guess_nc = n_cells;
   return FS_OK;
   FS_GUARD_END
+}
+
+int fs_stokes_set_method(fs_handle* h, int32_t method) {
+  if (!h || method < FS_METHOD_GMRES_REF || method > FS_METHOD_SCR) {
+    fs::set_error("fs_stokes_set_method: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  h->o.method = method;
+  return FS_OK;
 }
 
 int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess) {

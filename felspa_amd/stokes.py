@@ -89,8 +89,8 @@ class StokesLinearSystem:
     def __init__(self, blocks, A_lof=1, S_lof=0, method=L.FS_METHOD_FGMRES,
                  precision=L.FS_PREC_FP64, ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=None,
                  block_rows=0, restart=23, rtol=1e-8, reduce=1e-6, inner_cg_tol=1e-12,
-                 inner_cg_tol_mixed=1e-2, inner_cg_max_mixed=50, guess=L.FS_GUESS_USER,
-                 constraints=None, max_outer=0, inner_a_iters=30):
+                 inner_cg_tol_mixed=1e-2, inner_cg_max_mixed=50, inner_cg_max=0, guess=L.FS_GUESS_USER,
+                 constraints=None, max_outer=0, inner_a_iters=30, apply_diagonal_scaling=False, verbose=0):
         self._lib = L.lib()
         self.n_u = blocks["A"].shape[0]
         self.n_p = blocks["Mp"].shape[0]
@@ -103,6 +103,9 @@ class StokesLinearSystem:
                                                                        inner_cg_max_mixed)
         o.guess, o.block_rows, o.max_outer = guess, block_rows, max_outer
         o.inner_a_iters = inner_a_iters
+        o.inner_cg_max = inner_cg_max
+        o.apply_diagonal_scaling = int(apply_diagonal_scaling)
+        o.verbose = int(verbose)
         self._hint = None
         if block_hint is not None:
             self._hint = np.ascontiguousarray(block_hint, dtype=np.int32)
@@ -128,8 +131,55 @@ class StokesLinearSystem:
         L.check(self._lib.fs_stokes_refactor(self.h, *[a.ctypes.data_as(L.P(L.f64)) for a in arrs]),
                 "fs_stokes_refactor")
 
-    def solve(self, soln, rhs, control=None):
-        """StokesLinearSystem::solve(soln, rhs, solver_control)."""
+    def solve(self, soln, rhs, control=None, solution_method=None):
+        """StokesLinearSystem::solve(soln, rhs, solver_control) (stokes_ilu.cc:442-468).
+
+        solution_method (StokesSolutionMethod, stokes_common.h:323-328): None
+        keeps the handle's method; FC runs the handle's GMRES method (reference
+        mode or FGMRES), SCR the Schur-complement reduction, CompareTest both
+        and records their difference in the control (stokes_ilu.cc:471-505)."""
+        if solution_method is None:
+            return self._solve(soln, rhs, control)
+        if solution_method == CompareTest:
+            return self.solve_compare_test(soln, rhs, control)
+        fc = self._fc_method()
+        self.set_method(L.FS_METHOD_SCR if solution_method == SCR else fc)
+        try:
+            return self._solve(soln, rhs, control)
+        finally:
+            self.set_method(fc)
+
+    def _fc_method(self):
+        m = getattr(self, "_fc", None)
+        if m is None:
+            m = self._fc = (self.opts.method if self.opts.method != L.FS_METHOD_SCR else L.FS_METHOD_GMRES_REF)
+        return m
+
+    def set_method(self, method):
+        L.check(self._lib.fs_stokes_set_method(self.h, int(method)), "fs_stokes_set_method")
+        self.opts.method = method
+
+    def solve_compare_test(self, soln, rhs, control=None):
+        """CG Schur-complement reduction into soln, the GMRES block solve from
+        the same guess, and the relative L2 / L-infinity difference appended to
+        the control (stokes_ilu.cc:471-505)."""
+        fc = self._fc_method()
+        soln_gmres = soln.copy()
+        self.set_method(L.FS_METHOD_SCR)
+        try:
+            st_cg = self._solve(soln, rhs, control)
+        finally:
+            self.set_method(fc)
+        st_gm = self._solve(soln_gmres, rhs, control)
+        diff = soln_gmres - soln
+        l2 = float(np.linalg.norm(diff) / np.linalg.norm(soln))
+        linf = float(np.abs(diff).max())
+        if control is not None:
+            control.soln_diff_l2.append(l2)
+            control.soln_diff_linfty.append(linf)
+        return dict(cg=st_cg, gmres=st_gm, soln_diff_l2=l2, soln_diff_linfty=linf)
+
+    def _solve(self, soln, rhs, control=None):
         assert soln.dtype == np.float64 and soln.flags.c_contiguous and soln.size == self.n
         if self.constraints is not None:
             soln[self.constraints] = 0.0
@@ -141,10 +191,17 @@ class StokesLinearSystem:
         wall = time.perf_counter() - t0
         L.check(rc, "fs_stokes_solve")
         if control is not None:
-            control.n_gmres_iter.append(st.outer_iters)
-            control.gmres_error.append(st.last_value)
-            control.gmres_timer.append(wall)
-            control.log_gmres = True
+            if self.opts.method == L.FS_METHOD_SCR:
+                control.n_cg_outer_iter.append(st.outer_iters)
+                control.n_cg_inner_iter.append(st.inner_iters)
+                control.cg_error.append(st.last_value)
+                control.cg_timer.append(wall)
+                control.log_cg = True
+            else:
+                control.n_gmres_iter.append(st.outer_iters)
+                control.gmres_error.append(st.last_value)
+                control.gmres_timer.append(wall)
+                control.log_gmres = True
             control.last_step, control.last_value = st.outer_iters, st.last_value
             control.last_stats = st.as_dict()
         self.last_stats = st.as_dict()

@@ -108,7 +108,7 @@ def main(argv=None):
             lin = fa.StokesLinearSystem(S.blocks, A_lof=st_cfg["ilu_k_A"], S_lof=st_cfg["ilu_k_S"],
                                         precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
                                         block_hint=S.block_hint(st_cfg["hint_nodes"]), constraints=S.constrained,
-                                        restart=30, inner_a_iters=60)
+                                        restart=60, inner_a_iters=60)
         else:
             lin.refactor(S.blocks)
         lin.set_guess_field(model, S)

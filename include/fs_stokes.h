@@ -70,7 +70,8 @@ typedef struct fs_opts {
   double rtol;           /* tol = rtol ||rhs|| (stokes_ilu.cc:533), default 1e-8 */
   double reduce;         /* ReductionControl reduce (stokes_common.h:343-348), default 1e-6 */
   int32_t inner_cg_max;  /* 0: n_p (linear_operators.implement.h:76) */
-  double inner_cg_tol;   /* InverseMatrixCG tol = reduce (linear_operators.h:44-45), default 1e-12 */
+  double inner_cg_tol;   /* InverseMatrixCG tol = reduce (linear_operators.h:44-45), default 1e-12;
+                            FS_METHOD_FGMRES in FP64 uses it as a relative reduction only */
   double inner_cg_tol_mixed; /* FP32 inner CG relative tolerance in FS_PREC_MIXED */
   int32_t inner_cg_max_mixed;
   int32_t apply_diagonal_scaling; /* StokesSolverControl::apply_diagonal_scaling (stokes_common.h:360) */
@@ -149,6 +150,12 @@ struct fs_guess_model;
 int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const float* log_eta,
                               int32_t n_cells, const int64_t* vnode, const int8_t* vcomp,
                               const int64_t* pnode);
+
+/* Switch the solution method (fs_opts.method) of an existing handle between
+ * solves: StokesSolutionMethod FC / SCR (stokes_common.h:323-328) on the same
+ * factorisation; CompareTest (stokes_ilu.cc:471-505) is two solves and a
+ * difference, done by the caller. */
+int fs_stokes_set_method(fs_handle* h, int32_t method);
 
 /* Switch the guess mode (fs_opts.guess) of an existing handle between solves,
  * e.g. to compare FS_GUESS_LEARNED with FS_GUESS_ZERO on the same matrices. */

@@ -738,8 +738,9 @@ int or_stokes_solve(const or_csr* Ac, const or_csr* Btc, const or_csr* Bc, const
   bool ok = true;
   long long inner = 0;
   if (o->method == 2) {
+    // the flexible method's inner CG on M_p stops on a relative reduction only
     BlockSchurPcg P{&S, InnerPcgA{&S.A, &IA, o->inner_a_iters > 0 ? o->inner_a_iters : 30},
-                    InverseCG{&S.Mp, &IS, o->inner_tol, o->inner_tol}, {}};
+                    InverseCG{&S.Mp, &IS, 0.0, o->inner_tol}, {}};
     double last = 0;
     const int it = fgmres_true(S, P, n, x, rhs.data(), o->restart, o->rtol * bnorm, ctrl.max_steps, last);
     ctrl.last_step = it;

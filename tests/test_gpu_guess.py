@@ -203,7 +203,7 @@ def test_learned_guess_requires_a_field():
 def test_trained_network_gains_on_held_out_fields():
     """The committed network (trained on 512 seeded C4 fields) against the
     zero guess on held-out fields (samples 512+): same tolerance, fewer outer
-    iterations on average, and a guess residual below the zero guess's."""
+    iterations on average; a guess worse than zero is replaced by zero."""
     m = G.GuessModel.load(G.DEFAULT_2D)
     outer_l, outer_z, gres = [], [], []
     for s in (512, 513, 514, 515):
@@ -216,9 +216,13 @@ def test_trained_network_gains_on_held_out_fields():
         x = np.zeros(S.n)
         stl = lin.solve(x, S.rhs)
         assert stl["true_rel_residual"] <= 1e-8 and stz["true_rel_residual"] <= 1e-8
+        # the guess is used exactly when its residual is below the zero guess's
+        assert (stl["guess_used"] == L.FS_GUESS_LEARNED) == (stl["guess_rel_residual"] < 1.0)
+        if stl["guess_used"] == L.FS_GUESS_ZERO:
+            assert stl["outer_iters"] == stz["outer_iters"]
         outer_l.append(stl["outer_iters"])
         outer_z.append(stz["outer_iters"])
         gres.append(stl["guess_rel_residual"])
     print("outer learned", outer_l, "zero", outer_z, "guess residual", gres)
     assert np.mean(outer_l) < np.mean(outer_z)
-    assert max(gres) < 1.0
+    assert np.median(gres) < 0.5
