@@ -120,6 +120,10 @@ def main():
     ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start leg (C4 fields)")
     ap.add_argument("--learned-samples", type=int, default=4, help="held-out C4 fields in the learned leg")
+    ap.add_argument("--no-ref", action="store_true", help="skip the tight FP64 reference solve (error columns)")
+    ap.add_argument("--kernel-reps", type=int, default=30, help="launches per kernel class in the roofline timing")
+    ap.add_argument("--copy-probe-mb", type=float, default=0.0,
+                    help="also run a plain device copy of this many MB (HBM counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the solve is not sharded across GPUs "
@@ -180,17 +184,28 @@ def main():
     ms_mixed, st_mixed = timed(mixed, args.steps, args.warmup, "mixed")
     x_mixed = d_x.cpu().numpy().copy()
     out_extra = {}
+    if not args.no_plain:
+        # the same method with FP32-stored matrices and factors but the
+        # preconditioner's vectors and arithmetic in FP64
+        f64v = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_MIXED_FP64_VECTORS, guess=L.FS_GUESS_ZERO,
+                                     **common)
+        ms_f64v, st_f64v = timed(f64v, 1, 1, "mixed-fp64-vectors")
+        out_extra.update(mixed_fp64_vectors_ms=round(ms_f64v, 3),
+                         mixed_fp64_vectors_outer_iters=st_f64v["outer_iters"])
+        del f64v
     if plain is not None:
         ms_plain, st_plain = timed(plain, max(1, args.steps), min(args.warmup, 1), "plain-fp64")
         x_plain = d_x.cpu().numpy().copy()
         out_extra.update(plain_fp64_ms=round(ms_plain, 3), speedup_vs_plain_fp64=round(ms_plain / ms_mixed, 3),
                          plain_outer_iters=st_plain["outer_iters"], plain_inner_iters=st_plain["inner_iters"])
     # accuracy: against a tight FP64 GPU solve (rtol 1e-12) of the same system
-    ref_lin = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO, rtol=1e-12,
-                                    **{k: v for k, v in common.items()})
-    solve(ref_lin)
-    x_ref = d_x.cpu().numpy().copy()
-    del ref_lin
+    x_ref = None
+    if not args.no_ref:
+        ref_lin = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_FP64, guess=L.FS_GUESS_ZERO, rtol=1e-12,
+                                        **{k: v for k, v in common.items()})
+        solve(ref_lin)
+        x_ref = d_x.cpu().numpy().copy()
+        del ref_lin
 
     def errs(x):
         nu = S.n_u
@@ -201,11 +216,12 @@ def main():
         ep0 = np.linalg.norm(dp - dp.mean()) / np.linalg.norm(tp - tp.mean())
         return round(float(eu), 10), round(float(ep), 10), round(float(ep0), 10)
 
-    eu, ep, ep0 = errs(x_mixed)
-    out_extra.update(err_u_vs_tight=eu, err_p_vs_tight=ep, err_p_meanfree_vs_tight=ep0)
-    if plain is not None:
-        e = errs(x_plain)
-        out_extra.update(plain_err_u_vs_tight=e[0], plain_err_p_vs_tight=e[1])
+    if x_ref is not None:
+        eu, ep, ep0 = errs(x_mixed)
+        out_extra.update(err_u_vs_tight=eu, err_p_vs_tight=ep, err_p_meanfree_vs_tight=ep0)
+        if plain is not None:
+            e = errs(x_plain)
+            out_extra.update(plain_err_u_vs_tight=e[0], plain_err_p_vs_tight=e[1])
 
     # roofline of the dominant kernel class of the mixed solve, timed with HIP
     # events on the solver's own stream. Per mixed solve every inner PCG step
@@ -219,7 +235,8 @@ def main():
     launches = {0: 1, 2: 2 * n_stages, 1: 1, 3: 1}
     timing = {}
     for kind in (0, 2, 1, 3):
-        ms, by = mixed.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED, reps=30)
+        ms, by = mixed.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED,
+                                   reps=args.kernel_reps)
         timing[kind] = (ms, by)
         log("kernel %-52s %.4f ms/unit  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
     n_inner = st_mixed["inner_iters"]
@@ -228,13 +245,35 @@ def main():
     dom = 2 if share[2] >= share[0] else 0
     ms_dom, by_dom = timing[dom]
     achieved = by_dom / (ms_dom * 1e-3) / 1e9
+    # HBM bytes per unit from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes on this workload (scaled by a plain copy of known size)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath) and args.n_cells is None and args.ilu_k is None:
+        with open(tpath) as f:
+            tt = json.load(f).get(cfg, {}).get("ilu_apply" if dom == 2 else "spmv_A")
+        if tt:
+            traffic, traffic_src = tt["traffic_bytes"], tt["source"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kinds[dom],
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": kinds[dom],
                 "bytes_per_unit": by_dom, "ms_per_unit": round(ms_dom, 5), "launches_per_unit": launches[dom],
                 "share_of_solve": round(share[dom], 3)}
     out_extra["kernel_gbs"] = {kinds[k]: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for k, v in timing.items()}
     out_extra["kernel_ms_per_unit"] = {kinds[k]: round(v[0], 5) for k, v in timing.items()}
     out_extra["solve_gbs_algorithmic"] = round(st_mixed["bytes_algorithmic"] / (st_mixed["t_total_ms"] * 1e-3) / 1e9, 1)
+
+    if args.copy_probe_mb > 0:
+        # a plain device-to-device copy of known size: the HBM byte counters of
+        # a profile are scaled by what they read for it
+        nel = int(args.copy_probe_mb * 1e6 / 4)
+        a = torch.ones(nel, dtype=torch.float32, device=dev)
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        out_extra["copy_probe_bytes"] = 2 * 4 * nel
+        del a, c
 
     # learned warm start: held-out fields of the ellipse family (2D 128^2),
     # mixed + learned guess vs mixed + zero guess vs the plain FP64 solve

@@ -15,7 +15,7 @@ FS_OK, FS_NOT_CONVERGED = 0, 1
 FS_ERR_BAD_INPUT, FS_ERR_ZERO_PIVOT, FS_ERR_OUT_OF_MEMORY = -1, -2, -3
 FS_ERR_HIP, FS_ERR_UNSUPPORTED, FS_ERR_TIMEOUT, FS_ERR_NO_DEVICE = -4, -5, -6, -7
 FS_METHOD_GMRES_REF, FS_METHOD_FGMRES, FS_METHOD_SCR = 0, 1, 2
-FS_PREC_FP64, FS_PREC_MIXED = 0, 1
+FS_PREC_FP64, FS_PREC_MIXED, FS_PREC_MIXED_FP64_VECTORS = 0, 1, 2
 FS_ORDER_REFERENCE, FS_ORDER_BLOCK_COLOR = 0, 1
 FS_GUESS_USER, FS_GUESS_ZERO, FS_GUESS_LEARNED = 0, 1, 2
 FS_FIELD_INCLUSION, FS_FIELD_LAYERS, FS_FIELD_ELLIPSES, FS_FIELD_CONSTANT = 0, 1, 2, 3

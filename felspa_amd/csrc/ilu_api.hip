@@ -205,13 +205,13 @@ static int ilu_apply(fs_ilu* h, const double* src, double* dst, int32_t precisio
     float* t = a + ((n + 63) & ~int64_t(63));
     hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, n, h->va.as<double>(), a);
     if (transposed) h->FT.apply_transposed<float>(a, t, a, h->st, h->cnt);
-    else h->F.apply<float, float>(a, t, a, h->st, h->cnt);
+    else h->F.apply<float, float, float>(a, t, a, h->st, h->cnt);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(BLOCK), 0, h->st, n, (const float*)a,
                        h->va.as<double>());
   } else {
     if (transposed) h->FT.apply_transposed<double>(h->va.as<double>(), h->vb.as<double>(), h->va.as<double>(),
                                                    h->st, h->cnt);
-    else h->F.apply<double, double>(h->va.as<double>(), h->vb.as<double>(), h->va.as<double>(), h->st, h->cnt);
+    else h->F.apply<double, double, double>(h->va.as<double>(), h->vb.as<double>(), h->va.as<double>(), h->st, h->cnt);
   }
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, h->perm.as<int32_t>(),
                      h->va.as<double>(), h->vc.as<double>(), 1);

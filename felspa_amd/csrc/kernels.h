@@ -216,11 +216,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off,
   }
 }
 
-// q = A p on a CSR4 matrix; partial[b] = sum over the workgroup's rows of p_i q_i
-template <typename T, int G>
+// q = A p on a CSR4 matrix with TV values and T vectors / arithmetic;
+// partial[b] = sum over the workgroup's rows of p_i q_i
+template <typename TV, typename T, int G>
 __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_t* __restrict__ rp4,
                                                      const i32x4* __restrict__ ci4,
-                                                     const typename Vec4<T>::type* __restrict__ val4,
+                                                     const typename Vec4<TV>::type* __restrict__ val4,
                                                      const T* __restrict__ p, T* __restrict__ q,
                                                      double* __restrict__ partial) {
   __shared__ double sh[BLOCK / WAVE];
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
   const int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G;
   double contrib = 0;
   if (row < nrows) {
-    const T acc = row_dot4x2<T, T, T, G>(ci4, val4, p, rp4[row], rp4[row + 1], lane);
+    const T acc = row_dot4x2<TV, T, T, G>(ci4, val4, p, rp4[row], rp4[row + 1], lane);
     if (lane == 0) {
       q[row] = acc;
       contrib = (double)acc * (double)p[row];
@@ -677,8 +678,10 @@ struct DenseTri {
   const T* inv;
 };
 
-__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int ext_cap, int tsz) {
-  const size_t b = (size_t)tsz * (inv_cap + ext_cap + 256) + 4 * (size_t)ext_cap;
+// LDS of one dense stage workgroup: packed inverse and external values (tv
+// bytes each), the block's right-hand side (ta bytes), external columns
+__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int ext_cap, int tv, int ta) {
+  const size_t b = (size_t)tv * (inv_cap + ext_cap) + (size_t)ta * 256 + 4 * (size_t)ext_cap;
   return (b + 15) & ~(size_t)15;
 }
 
@@ -706,20 +709,20 @@ struct Vec16<double> {
 //     value * finished value of an earlier (forward) / later (backward) stage;
 //  3. G threads per row: y_i = sum_j inv(i, j) t_j from LDS.
 // Blocks with more than ext_cap external entries read them from global memory.
-template <typename T, typename TB, typename TY, int G, bool FWD>
-__global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<T> F, int blk0, int inv_cap, int ext_cap,
+template <typename TV, typename T, typename TB, typename TY, int G, bool FWD>
+__global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<TV> F, int blk0, int inv_cap, int ext_cap,
                                                            const TB* b, TY* y) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
-  using V = typename Vec16<T>::type;
-  constexpr int VN = Vec16<T>::n;
-  T* Ms = (T*)dsm;
-  T* ev = Ms + inv_cap;
-  T* t = ev + ext_cap;
+  using V = typename Vec16<TV>::type;
+  constexpr int VN = Vec16<TV>::n;
+  TV* Ms = (TV*)dsm;
+  TV* ev = Ms + inv_cap;
+  T* t = (T*)(ev + ext_cap);  // inv_cap, ext_cap: multiples of 4, so 16-byte aligned
   int32_t* ec = (int32_t*)(t + 256);
   const int blk = blk0 + blockIdx.x;
   const int tid = threadIdx.x;
   const int r0 = F.blk_ptr[blk], nr = F.blk_ptr[blk + 1] - r0;
-  const T* M = F.inv + F.inv_off[blk];
+  const TV* M = F.inv + F.inv_off[blk];
   const int msz = nr * (nr + 1) / 2;
   const int64_t e0 = F.xptr[r0];
   const int ne = (int)(F.xptr[r0 + nr] - e0);  // a multiple of 4
@@ -740,18 +743,18 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<T> F, int bl
     const int p0 = (int)(F.xptr[r] - e0), p1 = (int)(F.xptr[r + 1] - e0);
     T acc = 0;
     for (int p = p0 + gl; p < p1; p += 4 * G) {
-      T v[4];
+      TV v[4];
       TY xg[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int q0 = p + u * G;
         const int q = q0 < p1 ? q0 : p1 - 1;
         const int32_t c = staged ? ec[q] : F.xci[e0 + q];
-        v[u] = q0 < p1 ? (staged ? ev[q] : F.xv[e0 + q]) : (T)0;
+        v[u] = q0 < p1 ? (staged ? ev[q] : F.xv[e0 + q]) : (TV)0;
         xg[u] = y[c];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc += v[u] * (T)xg[u];
+      for (int u = 0; u < 4; ++u) acc += (T)v[u] * (T)xg[u];
     }
     acc = group_reduce<G>(acc);
     if (gl == 0) t[i] = (T)b[r] - acc;
@@ -760,9 +763,9 @@ __global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<T> F, int bl
   for (int i = grp; i < nr; i += BLOCK / G) {
     T acc = 0;
     if (FWD) {  // columns j <= i; column j starts at j nr - j (j - 1) / 2
-      for (int j = gl; j <= i; j += G) acc += Ms[j * nr - j * (j - 1) / 2 + i - j] * t[j];
+      for (int j = gl; j <= i; j += G) acc += (T)Ms[j * nr - j * (j - 1) / 2 + i - j] * t[j];
     } else {    // columns j >= i; column j starts at j (j + 1) / 2
-      for (int j = i + gl; j < nr; j += G) acc += Ms[j * (j + 1) / 2 + i] * t[j];
+      for (int j = i + gl; j < nr; j += G) acc += (T)Ms[j * (j + 1) / 2 + i] * t[j];
     }
     acc = group_reduce<G>(acc);
     if (gl == 0) y[r0 + i] = (TY)acc;

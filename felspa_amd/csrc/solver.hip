@@ -463,69 +463,74 @@ static size_t dense_wg_lds_limit() {
   return lim;
 }
 
-template <typename T, typename TB, typename TY, int G, bool FWD>
+template <typename TV, typename T, typename TB, typename TY, int G, bool FWD>
 static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
   const int nb = b1 - b0;
   if (nb <= 0) return;
-  const bool d = std::is_same<T, double>::value;
-  DenseTri<T> D;
+  const bool d = std::is_same<TV, double>::value;
+  DenseTri<TV> D;
   D.blk_ptr = F.blk_ptr.as<int32_t>();
   D.xptr = FWD ? F.xlptr.as<int64_t>() : F.xuptr.as<int64_t>();
   D.xci = FWD ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
-  D.xv = FWD ? (d ? (const T*)F.xlv64.p : (const T*)F.xlv32.p) : (d ? (const T*)F.xuv64.p : (const T*)F.xuv32.p);
+  D.xv = FWD ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
   D.inv_off = F.inv_off.as<int64_t>();
-  D.inv = FWD ? (d ? (const T*)F.linv64.p : (const T*)F.linv32.p) : (d ? (const T*)F.uinv64.p : (const T*)F.uinv32.p);
+  D.inv = FWD ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
   // external entries staged in LDS up to what the workgroup's LDS allows
   const size_t lim = dense_wg_lds_limit();
-  const size_t fixed = dense_lds_bytes(F.inv_cap, 0, sizeof(T));
+  const size_t fixed = dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T));
   const int want = FWD ? F.xl_max : F.xu_max;
-  int ext_cap = lim > fixed + 64 ? (int)std::min<size_t>(want, (lim - fixed - 16) / (sizeof(T) + 4)) : 0;
+  int ext_cap = lim > fixed + 64 ? (int)std::min<size_t>(want, (lim - fixed - 16) / (sizeof(TV) + 4)) : 0;
   ext_cap &= ~3;
-  const size_t lds = dense_lds_bytes(F.inv_cap, ext_cap, sizeof(T));
-  auto fn = &k_ilu_stage_dense<T, TB, TY, G, FWD>;
+  const size_t lds = dense_lds_bytes(F.inv_cap, ext_cap, sizeof(TV), sizeof(T));
+  auto fn = &k_ilu_stage_dense<TV, T, TB, TY, G, FWD>;
   if (lds > 64 * 1024)
     hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
               "hipFuncSetAttribute");
   hipLaunchKernelGGL(fn, dim3(nb), dim3(BLOCK), lds, st, D, b0, F.inv_cap, ext_cap, b, y);
 }
 
-template <typename T, typename TB, typename TY, bool FWD>
+template <typename TV, typename T, typename TB, typename TY, bool FWD>
 static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
   // threads per row: the block's rows fill the 256-thread workgroup
   const int mr = F.sched.max_block_rows;
-  if (mr <= 32) launch_dense<T, TB, TY, 8, FWD>(F, b0, b1, b, y, st);
-  else if (mr <= 64) launch_dense<T, TB, TY, 4, FWD>(F, b0, b1, b, y, st);
-  else if (mr <= 128) launch_dense<T, TB, TY, 2, FWD>(F, b0, b1, b, y, st);
-  else launch_dense<T, TB, TY, 1, FWD>(F, b0, b1, b, y, st);
+  if (mr <= 32) launch_dense<TV, T, TB, TY, 8, FWD>(F, b0, b1, b, y, st);
+  else if (mr <= 64) launch_dense<TV, T, TB, TY, 4, FWD>(F, b0, b1, b, y, st);
+  else if (mr <= 128) launch_dense<TV, T, TB, TY, 2, FWD>(F, b0, b1, b, y, st);
+  else launch_dense<TV, T, TB, TY, 1, FWD>(F, b0, b1, b, y, st);
 }
 
-// x = (LU)^{-1} b : forward into tmp, backward into x (tmp may alias x)
-template <typename T, typename TB>
+// x = (LU)^{-1} b with TV factor values and T vectors / arithmetic: forward
+// into tmp, backward into x (tmp may alias x)
+template <typename TV, typename T, typename TB>
 void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
   const int ns = sched.n_stages();
   // the dense kernels need a block's packed triangle in LDS
-  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(T)) + 64 <= dense_wg_lds_limit()) {
+  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(TV), sizeof(T)) + 64 <= dense_wg_lds_limit()) {
     for (int s = 0; s < ns; ++s)
-      launch_dense_g<T, TB, T, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
+      launch_dense_g<TV, T, TB, T, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
     for (int s = ns - 1; s >= 0; --s)
-      launch_dense_g<T, T, T, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
+      launch_dense_g<TV, T, T, T, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
     cnt.launches += 2 * ns;
     // external entries (value + column), both packed triangles, the row
     // pointers of the external spans, b read, y written, gathered values
     const double ext = (double)(n_xl + n_xu);
-    cnt.bytes += ext * (sizeof(T) + 4) + 2.0 * inv_total * sizeof(T) + 16.0 * n + 4.0 * n * sizeof(T);
+    cnt.bytes += ext * (sizeof(TV) + 4) + 2.0 * inv_total * sizeof(TV) + 16.0 * n + 4.0 * n * sizeof(T);
     return;
   }
+  if (!std::is_same<TV, T>::value)
+    throw FsError(FS_ERR_UNSUPPORTED, "ILU application with FP32 factors and FP64 vectors needs the dense kernels");
+  using TS = T;  // substitution kernels: factor values and vectors of one type
   for (int s = 0; s < ns; ++s)
-    launch_stage_g<T, TB, T, true, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
+    launch_stage_g<TS, TB, TS, true, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
   for (int s = ns - 1; s >= 0; --s)
-    launch_stage_g<T, T, T, false, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
+    launch_stage_g<TS, TS, TS, false, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
   cnt.launches += 2 * ns;
   cnt.bytes += (double)nnz * (sizeof(T) + 4) + 16.0 * (n + 1) + 3.0 * n * sizeof(T) + 20.0 * n;
 }
-template void IluDev::apply<double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
-template void IluDev::apply<float, double>(const double*, float*, float*, hipStream_t, Counters&) const;
-template void IluDev::apply<float, float>(const float*, float*, float*, hipStream_t, Counters&) const;
+template void IluDev::apply<double, double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
+template void IluDev::apply<float, double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
+template void IluDev::apply<float, float, double>(const double*, float*, float*, hipStream_t, Counters&) const;
+template void IluDev::apply<float, float, float>(const float*, float*, float*, hipStream_t, Counters&) const;
 
 // transposed solve with a transposed factor (lower part = U^T with diagonal,
 // upper part = L^T unit)
@@ -644,6 +649,9 @@ template void spmv4_launch<double, double, double, double, double, 1>(const DevC
 template void spmv4_launch<double, double, double, double, double, 2>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
 template void spmv4_launch<float, float, double, double, float, 0>(const DevCsr&, int64_t, int64_t, const float*, double*, const double*, int64_t, hipStream_t);
 template void spmv4_launch<float, float, float, double, float, 2>(const DevCsr&, int64_t, int64_t, const float*, float*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<float, double, double, double, double, 0>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<double, float, double, double, double, 0>(const DevCsr&, int64_t, int64_t, const float*, double*, const double*, int64_t, hipStream_t);
+template void spmv4_launch<float, double, double, double, double, 2>(const DevCsr&, int64_t, int64_t, const double*, double*, const double*, int64_t, hipStream_t);
 
 }  // namespace fs
 

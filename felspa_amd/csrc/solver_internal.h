@@ -85,7 +85,7 @@ struct IluDev {
   fsk::IluView<T> view() const;
   void factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit, hipStream_t st,
                  int* d_err, Counters& cnt);
-  template <typename T, typename TB>
+  template <typename TV, typename T, typename TB>
   void apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;
   template <typename T>
   void apply_transposed(const T* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;

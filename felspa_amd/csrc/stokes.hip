@@ -245,7 +245,7 @@ struct fs_handle {
     a.d = w + 2 * lp;
     a.q = w + 3 * lp;
     a.partial = dptr<double>(partial);
-    const bool mixed = std::is_same<T, float>::value;
+    const bool mixed = o.precision != FS_PREC_FP64;
     // InverseMatrixCG's ReductionControl(tol, reduce) for the reference-mode
     // GMRES and SCR; the flexible method only needs a relative reduction
     a.tol = (mixed || o.method == FS_METHOD_FGMRES) ? 0.0 : o.inner_cg_tol;
@@ -288,7 +288,7 @@ struct fs_handle {
   // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0.
   // Scalars never leave the device: the update / direction kernels sum the
   // producers' per-workgroup partials themselves (fixed order).
-  template <typename T>
+  template <typename TV, typename T>
   void inner_pcg_A(const double* src, T* x, int iters) {
     T* base = (T*)pcg.p;
     T *r = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
@@ -297,20 +297,20 @@ struct fs_handle {
     double* rz = S(SC_PCG + 1);  // two slots
     const int gv = 512, gu = 256;
     hipLaunchKernelGGL((k_copy_convert<double, T>), dim3(gv), dim3(BLOCK), 0, st, nu, src, r);
-    FA.apply<T, T>(r, y, z, st, cnt);
+    FA.apply<TV, T, T>(r, y, z, st, cnt);
     hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
     hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
     cnt.launches += 3;
     int cur = 0;
     for (int it = 0; it < iters; ++it) {
-      launch_spmv_dot<T>(pv, q, part_pq);
+      launch_spmv_dot<TV, T>(pv, q, part_pq);
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
       hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
                          it == 0 ? 1 : 0);
       cnt.launches += 3;
-      cnt.bytes += Ablk.nnz * (sizeof(T) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
+      cnt.bytes += Ablk.nnz * (sizeof(TV) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
       if (it + 1 == iters) break;
-      FA.apply<T, T>(r, y, z, st, cnt);
+      FA.apply<TV, T, T>(r, y, z, st, cnt);
       hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
       hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
                          rz + (cur ^ 1), z, pv, 0);
@@ -323,15 +323,15 @@ struct fs_handle {
   int spmv_dot_grid() const { return (int)std::max<int64_t>(1, (nu * Ablk.G4 + BLOCK - 1) / BLOCK); }
 
   // q = A p and the per-workgroup partials of p.q, on the padded A block;
-  // av selects the precision (Ablk.v4_32 or Ablk.v4_64)
-  template <typename T>
+  // TV selects the stored values (Ablk.v4_32 or Ablk.v4_64), T the vectors
+  template <typename TV, typename T>
   void launch_spmv_dot(const T* pv, T* q, double* part) {
     const int g = spmv_dot_grid();
-    using V4 = typename Vec4<T>::type;
-    const V4* av = std::is_same<T, double>::value ? (const V4*)Ablk.v4_64.p : (const V4*)Ablk.v4_32.p;
+    using V4 = typename Vec4<TV>::type;
+    const V4* av = std::is_same<TV, double>::value ? (const V4*)Ablk.v4_64.p : (const V4*)Ablk.v4_32.p;
 #define FS_SD(GG)                                                                                            \
   case GG:                                                                                                   \
-    hipLaunchKernelGGL((k_spmv4_dot<T, GG>), dim3(g), dim3(BLOCK), 0, st, nu, Ablk.rp4.as<int64_t>(),         \
+    hipLaunchKernelGGL((k_spmv4_dot<TV, T, GG>), dim3(g), dim3(BLOCK), 0, st, nu, Ablk.rp4.as<int64_t>(),     \
                        Ablk.ci4.as<i32x4>(), av, pv, q, part);                                              \
     break;
     switch (Ablk.G4) {
@@ -341,13 +341,14 @@ struct fs_handle {
 #undef FS_SD
   }
 
-  template <typename T>
+  // TV: stored matrix / factor values; T: vectors and arithmetic
+  template <typename TV, typename T>
   void precond(const double* src, T* dst) {
     T* y = (T*)ytmp.p;
     const int ia = o.method == FS_METHOD_GMRES_REF ? 0 : o.inner_a_iters;
-    if (ia > 0) inner_pcg_A<T>(src, dst, ia);
-    else FA.apply<T, double>(src, y, dst, st, cnt);
-    spmv4_launch<T, T, T, double, T, 2>(K, np, nu, dst, (T*)tp.p, src + nu, nu, st);
+    if (ia > 0) inner_pcg_A<TV, T>(src, dst, ia);
+    else FA.apply<TV, T, double>(src, y, dst, st, cnt);
+    spmv4_launch<TV, T, T, double, T, 2>(K, np, nu, dst, (T*)tp.p, src + nu, nu, st);
     cnt.launches++;
     inner_cg<T>((const T*)tp.p, dst + nu);
   }
@@ -391,8 +392,9 @@ struct fs_handle {
     cnt.bytes += (double)n * (16.0 + dim * sizeof(TZ));
   }
 
-  // right-preconditioned flexible GMRES, FP64 true residual at each restart
-  template <typename T>
+  // right-preconditioned flexible GMRES, FP64 true residual at each restart;
+  // the preconditioner streams TV matrix / factor values and computes in T
+  template <typename TV, typename T>
   void fgmres(double tol_abs, int max_steps, fs_stats& s) {
     const int m = o.restart;
     double* Vb = dptr<double>(V);
@@ -415,9 +417,12 @@ struct fs_handle {
       for (int j = 0; j < m && acc < max_steps; ++j) {
         ++acc;
         T* zj = Zb + (int64_t)j * ld;
-        prec<T>(Vb + (int64_t)j * ld, zj);
+        prec<TV, T>(Vb + (int64_t)j * ld, zj);
         double* w = Vb + (int64_t)(j + 1) * ld;
-        kop_t<T>(zj, w);
+        // the Arnoldi relation holds for K in FP64: only the (flexible)
+        // preconditioner is approximate, so the restarts' FP64 true residual
+        // and the minimised residual belong to the same operator
+        kop_t<double, T>(zj, w);
         arnoldi_tail(j);
         dim = j + 1;
         est = read_scalar(SC_EST);
@@ -447,7 +452,7 @@ struct fs_handle {
     Ctrl::State stt = Ctrl::ITERATE;
     do {
       kop(dptr<double>(x), p, dptr<double>(b));
-      prec<double>(p, Vb);
+      prec<double, double>(p, Vb);
       norm(Vb, n, SC_NRM);
       const double rho = read_scalar(SC_NRM);
       stt = c.check(acc, rho);
@@ -457,7 +462,7 @@ struct fs_handle {
       for (int j = 0; j < m && stt == Ctrl::ITERATE; ++j) {
         ++acc;
         kop(Vb + (int64_t)j * ld, p, nullptr);
-        prec<double>(p, Vb + (int64_t)(j + 1) * ld);
+        prec<double, double>(p, Vb + (int64_t)(j + 1) * ld);
         arnoldi_tail(j);
         dim = j + 1;
         stt = c.check(acc, read_scalar(SC_EST));
@@ -575,29 +580,29 @@ struct fs_handle {
     cnt.launches += 2;
     cnt.bytes += 40.0 * n;
   }
-  // w = K~ z (z in the preconditioned-vector precision)
-  template <typename T>
+  // w = K~ z (TV matrix values, z in the preconditioned-vector precision T)
+  template <typename TV, typename T>
   void kop_t(const T* z, double* w) {
     if (!Dsc) {
-      spmv4_launch<T, T, double, double, T, 0>(K, n, 0, z, w, (const double*)nullptr, 0, st);
+      spmv4_launch<TV, T, double, double, double, 0>(K, n, 0, z, w, (const double*)nullptr, 0, st);
       cnt.launches++;
-      cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
+      cnt.bytes += K.nnz * (sizeof(TV) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
       return;
     }
     T* t = (T*)dsc_tT();
     hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, z, Dsc, t, 1);
-    spmv4_launch<T, T, double, double, T, 0>(K, n, 0, t, w, (const double*)nullptr, 0, st);
+    spmv4_launch<TV, T, double, double, double, 0>(K, n, 0, t, w, (const double*)nullptr, 0, st);
     hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, w, Dsc, w, 1);
     cnt.launches += 3;
-    cnt.bytes += K.nnz * (sizeof(T) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0) + n * (2.0 * sizeof(T) + 32.0);
+    cnt.bytes += K.nnz * (sizeof(TV) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0) + n * (2.0 * sizeof(T) + 32.0);
   }
   // dst = P~^-1 src
-  template <typename T>
+  template <typename TV, typename T>
   void prec(const double* src, T* dst) {
-    if (!Dsc) { precond<T>(src, dst); return; }
+    if (!Dsc) { precond<TV, T>(src, dst); return; }
     double* t = dsc_t();
     hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, src, Dsc, t, 0);
-    precond<T>(t, dst);
+    precond<TV, T>(t, dst);
     hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, dst, Dsc, dst, 0);
     cnt.launches += 2;
     cnt.bytes += n * (24.0 + 2.0 * sizeof(T) + 8.0);
@@ -662,14 +667,14 @@ struct fs_handle {
     Ctrl::State stt = c.check(0, read_scalar(SC_TMP + 2));
     int it = 0, cur = 0;
     if (stt == Ctrl::ITERATE) {
-      FA.apply<double, double>(rr, y, z, st, cnt);
+      FA.apply<double, double, double>(rr, y, z, st, cnt);
       hipLaunchKernelGGL((k_dot_partial_t<double>), dim3(gv), dim3(BLOCK), 0, st, nu, rr, z, part_rz);
       hipLaunchKernelGGL((k_pcg_direction<double>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
       cnt.launches += 2;
     }
     while (stt == Ctrl::ITERATE) {
       ++it;
-      launch_spmv_dot<double>(pv, q, part_pq);
+      launch_spmv_dot<double, double>(pv, q, part_pq);
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
       hipLaunchKernelGGL((k_pcg_update<double>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, xv, rr, 0);
       cnt.launches += 3;
@@ -678,7 +683,7 @@ struct fs_handle {
       const double res = read_scalar(SC_TMP + 2);
       stt = c.check(it, res);
       if (stt != Ctrl::ITERATE) break;
-      FA.apply<double, double>(rr, y, z, st, cnt);
+      FA.apply<double, double, double>(rr, y, z, st, cnt);
       hipLaunchKernelGGL((k_dot_partial_t<double>), dim3(gv), dim3(BLOCK), 0, st, nu, rr, z, part_rz);
       hipLaunchKernelGGL((k_pcg_direction<double>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
                          rz + (cur ^ 1), z, pv, 0);
@@ -876,7 +881,7 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   cnt = Counters{};
   if (o.apply_diagonal_scaling && o.method == FS_METHOD_SCR)
     throw FsError(FS_ERR_UNSUPPORTED, "diagonal scaling with SCR is not implemented on the GPU path");
-  if (o.precision == FS_PREC_MIXED && (o.method == FS_METHOD_GMRES_REF || o.method == FS_METHOD_SCR))
+  if (o.precision != FS_PREC_FP64 && (o.method == FS_METHOD_GMRES_REF || o.method == FS_METHOD_SCR))
     throw FsError(FS_ERR_UNSUPPORTED, "reference-mode GMRES and SCR run in FP64 only");
   hipEvent_t e0, e1, e2;
   hip_check(hipEventCreate(&e0), "event");
@@ -929,8 +934,9 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   } else if (o.method == FS_METHOD_GMRES_REF) {
     gmres_ref(bnorm_k, max_steps, s);
   } else if (o.method == FS_METHOD_FGMRES) {
-    if (o.precision == FS_PREC_MIXED) fgmres<float>(o.rtol * bnorm_k, max_steps, s);
-    else fgmres<double>(o.rtol * bnorm_k, max_steps, s);
+    if (o.precision == FS_PREC_MIXED) fgmres<float, float>(o.rtol * bnorm_k, max_steps, s);
+    else if (o.precision == FS_PREC_MIXED_FP64_VECTORS) fgmres<float, double>(o.rtol * bnorm_k, max_steps, s);
+    else fgmres<double, double>(o.rtol * bnorm_k, max_steps, s);
     if (s.initial_rel_residual > 0) s.initial_rel_residual /= bnorm_k;
   } else if (o.method == FS_METHOD_SCR) {
     scr(s);
@@ -979,6 +985,8 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   else fs_opts_default(&h.o);
   auto& o = h.o;
   if (o.restart < 1 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [1, 63]");
+  if (o.precision < FS_PREC_FP64 || o.precision > FS_PREC_MIXED_FP64_VECTORS)
+    throw FsError(FS_ERR_BAD_INPUT, "unknown precision");
   if (o.ilu_k_A < 0 || o.ilu_k_S < 0 || o.ilu_k_A > 20 || o.ilu_k_S > 20)
     throw FsError(FS_ERR_BAD_INPUT, "level of fill out of range");
   if (!(o.rtol > 0)) throw FsError(FS_ERR_BAD_INPUT, "rtol must be positive");
@@ -1142,7 +1150,7 @@ int fs_stokes_vmult(fs_handle* h, const double* src, double* dst, int32_t precis
   const int32_t* pm = h->perm_all.as<int32_t>();
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->vin.as<double>(),
                      h->b.as<double>(), 0);
-  if (precision == FS_PREC_MIXED) {
+  if (precision != FS_PREC_FP64) {
     hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, n, h->b.as<double>(),
                        (float*)h->ytmp.p);
     spmv4_launch<float, float, double, double, float, 0>(h->K, n, 0, (const float*)h->ytmp.p, h->r.as<double>(),
@@ -1171,11 +1179,14 @@ int fs_stokes_precond_vmult(fs_handle* h, const double* src, double* dst, int32_
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->vin.as<double>(),
                      h->b.as<double>(), 0);
   if (precision == FS_PREC_MIXED) {
-    h->precond<float>(h->b.as<double>(), (float*)h->Z.p);
+    if (h->o.precision != FS_PREC_MIXED) throw FsError(FS_ERR_BAD_INPUT, "FP32 vectors need a FS_PREC_MIXED handle");
+    h->precond<float, float>(h->b.as<double>(), (float*)h->Z.p);
     hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(BLOCK), 0, h->st, n, (const float*)h->Z.p,
                        h->r.as<double>());
+  } else if (precision == FS_PREC_MIXED_FP64_VECTORS) {
+    h->precond<float, double>(h->b.as<double>(), h->r.as<double>());
   } else {
-    h->precond<double>(h->b.as<double>(), h->r.as<double>());
+    h->precond<double, double>(h->b.as<double>(), h->r.as<double>());
   }
   hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, pm, h->r.as<double>(),
                      h->vin.as<double>(), 1);
@@ -1273,7 +1284,10 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
   if (!h || reps < 1) { fs::set_error("fs_stokes_time_kernel: bad argument"); return FS_ERR_BAD_INPUT; }
   FS_GUARD_BEGIN
   if (h->dirty) h->numeric_factorize();
-  const bool mixed = precision == FS_PREC_MIXED;
+  // FS_PREC_MIXED: FP32 values and vectors; FS_PREC_MIXED_FP64_VECTORS: FP32
+  // values, FP64 vectors
+  const bool mixed = precision == FS_PREC_MIXED_FP64_VECTORS;
+  const bool f32 = precision == FS_PREC_MIXED;
   const int64_t nu = h->nu;
   // deterministic nonzero inputs in the work vectors
   const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (h->n + BLOCK - 1) / BLOCK));
@@ -1286,24 +1300,31 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
     h->cnt = Counters{};
     switch (kind) {
       case 0: {
-        if (mixed)
-          h->launch_spmv_dot<float>((const float*)h->ytmp.p, (float*)base,
-                                    dptr<double>(h->pcg_part));
+        if (f32)
+          h->launch_spmv_dot<float, float>((const float*)h->ytmp.p, (float*)base, dptr<double>(h->pcg_part));
+        else if (mixed)
+          h->launch_spmv_dot<float, double>(h->vin.as<double>(), (double*)base, dptr<double>(h->pcg_part));
         else
-          h->launch_spmv_dot<double>(h->vin.as<double>(), (double*)base,
-                                     dptr<double>(h->pcg_part));
-        h->cnt.bytes = h->Ablk.nnz * ((mixed ? 4.0 : 8.0) + 4.0) + 8.0 * (nu + 1) + (mixed ? 4.0 : 8.0) * 2.0 * nu;
+          h->launch_spmv_dot<double, double>(h->vin.as<double>(), (double*)base, dptr<double>(h->pcg_part));
+        const double sv = (mixed || f32) ? 4.0 : 8.0, sx = f32 ? 4.0 : 8.0;
+        h->cnt.bytes = h->Ablk.nnz * (sv + 4.0) + 8.0 * (nu + 1) + sx * 2.0 * nu;
         break;
       }
       case 1:
         h->kmult64(h->vin.as<double>(), h->r.as<double>(), nullptr);
         break;
       case 2:
-        if (mixed) h->FA.apply<float, float>((const float*)h->ytmp.p, (float*)base, (float*)base + h->ldu, h->st, h->cnt);
-        else h->FA.apply<double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st, h->cnt);
+        if (f32)
+          h->FA.apply<float, float, float>((const float*)h->ytmp.p, (float*)base, (float*)base + h->ldu, h->st, h->cnt);
+        else if (mixed)
+          h->FA.apply<float, double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st,
+                                             h->cnt);
+        else
+          h->FA.apply<double, double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st,
+                                              h->cnt);
         break;
       case 3:
-        if (mixed) h->inner_cg<float>((const float*)h->ytmp.p + nu, (float*)base);
+        if (f32) h->inner_cg<float>((const float*)h->ytmp.p + nu, (float*)base);
         else h->inner_cg<double>(h->vin.as<double>() + nu, (double*)base);
         break;
       default:

@@ -44,7 +44,15 @@ enum {
                               FP64 true residual ||b-Kx|| <= rtol ||b|| */
   FS_METHOD_SCR = 2        /* Schur-complement reduction (stokes_ilu.cc:564-710) */
 };
-enum { FS_PREC_FP64 = 0, FS_PREC_MIXED = 1 };
+enum {
+  FS_PREC_FP64 = 0,              /* everything in FP64 */
+  FS_PREC_MIXED = 1,             /* preconditioner in FP32: matrix blocks and ILU factors stored in
+                                    FP32, inner PCG / ILU application / inner CG computed in FP32;
+                                    outer FGMRES in FP64, including its products with K (the
+                                    restarts make it iterative refinement) */
+  FS_PREC_MIXED_FP64_VECTORS = 2 /* as FS_PREC_MIXED with the preconditioner's vectors and
+                                    arithmetic in FP64 (stored values FP32) */
+};
 enum {
   FS_ORDER_REFERENCE = 0,   /* ILU on the caller's row order (sequential stages) */
   FS_ORDER_BLOCK_COLOR = 1  /* ILU on a block-coloured symmetric permutation */

@@ -77,7 +77,7 @@ def test_reference_mode_matches_oracle(oracle, cfg, n):
     assert du < 1e-7 and dp < 1e-7, (du, dp)
 
 
-@pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED])
+@pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED, L.FS_PREC_MIXED_FP64_VECTORS])
 @pytest.mark.parametrize("cfg,n,kA", CASES)
 def test_fgmres_meets_true_residual(cfg, n, kA, precision):
     S = system(cfg, n)
@@ -113,7 +113,7 @@ def test_fgmres_matches_oracle_port(oracle, cfg, n, kA):
     assert du < 1e-6 and dpm < 1e-4, (du, dpm)
 
 
-@pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED])
+@pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED, L.FS_PREC_MIXED_FP64_VECTORS])
 @pytest.mark.parametrize("cfg,n,kA", CASES)
 def test_fgmres_tight_tolerance(cfg, n, kA, precision):
     S = system(cfg, n)
