@@ -670,105 +670,228 @@ __host__ __device__ inline int64_t packed_tri_size(int64_t nr) { return nr * (nr
 // are one contiguous span), and the packed inverse of its in-block triangle.
 template <typename T>
 struct DenseTri {
-  const int32_t* blk_ptr;
-  const int64_t* xptr;
-  const int32_t* xci;
-  const T* xv;
-  const int64_t* inv_off;
-  const T* inv;
+  const i32x4* desc;      // per block: first row, rows, first chunk, chunks
+  const int32_t* rng;     // per row: first chunk in the block | chunks << 16
+  const int32_t* xci;     // external columns, chunks of 4
+  const T* xv;            // external values, chunks of 4
+  const int64_t* inv_off; // per block: start of its packed triangle(s)
+  const T* inv;           // packed inverse of the in-block factor
 };
 
-// LDS of one dense stage workgroup: packed inverse and external values (tv
-// bytes each), the block's right-hand side (ta bytes), external columns
-__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int ext_cap, int tv, int ta) {
-  const size_t b = (size_t)tv * (inv_cap + ext_cap) + (size_t)ta * 256 + 4 * (size_t)ext_cap;
-  return (b + 15) & ~(size_t)15;
+// Lane gl's share of row i of a packed inverse times t, columns gl, gl + G,
+// ...: lower (columns j <= i, column j starting at j n - j (j - 1) / 2) and
+// upper (columns j >= i, column j starting at j (j + 1) / 2); four
+// independent partial sums so that the LDS reads overlap
+template <typename TV, typename T, int G>
+__device__ __forceinline__ T lower_row_dot(const TV* M, const T* t, int n, int i, int gl) {
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int j = gl;
+  for (; j + 3 * G <= i; j += 4 * G) {
+    const int j1 = j + G, j2 = j + 2 * G, j3 = j + 3 * G;
+    a0 += (T)M[j * n - j * (j - 1) / 2 + i - j] * t[j];
+    a1 += (T)M[j1 * n - j1 * (j1 - 1) / 2 + i - j1] * t[j1];
+    a2 += (T)M[j2 * n - j2 * (j2 - 1) / 2 + i - j2] * t[j2];
+    a3 += (T)M[j3 * n - j3 * (j3 - 1) / 2 + i - j3] * t[j3];
+  }
+  for (; j <= i; j += G) a0 += (T)M[j * n - j * (j - 1) / 2 + i - j] * t[j];
+  return (a0 + a1) + (a2 + a3);
+}
+template <typename TV, typename T, int G>
+__device__ __forceinline__ T upper_row_dot(const TV* M, const T* t, int n, int i, int gl) {
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int j = i + gl;
+  for (; j + 3 * G < n; j += 4 * G) {
+    const int j1 = j + G, j2 = j + 2 * G, j3 = j + 3 * G;
+    a0 += (T)M[j * (j + 1) / 2 + i] * t[j];
+    a1 += (T)M[j1 * (j1 + 1) / 2 + i] * t[j1];
+    a2 += (T)M[j2 * (j2 + 1) / 2 + i] * t[j2];
+    a3 += (T)M[j3 * (j3 + 1) / 2 + i] * t[j3];
+  }
+  for (; j < n; j += G) a0 += (T)M[j * (j + 1) / 2 + i] * t[j];
+  return (a0 + a1) + (a2 + a3);
 }
 
-// 16-byte vector of T
-template <typename T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-  typedef float type __attribute__((ext_vector_type(4)));
-  static constexpr int n = 4;
-};
-template <>
-struct Vec16<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-  static constexpr int n = 2;
-};
+// Dense stage kernels: MODE 0 forward stage, 1 backward stage, 2 the last
+// stage forward then backward in one launch (its rows have no external U
+// entries: every later column of a last-stage row lies in its own block).
+enum { DS_FWD = 0, DS_BWD = 1, DS_BOTH = 2 };
 
-// One stage of the forward (FWD) or backward solve, one workgroup per block.
-// Packed triangles start at multiples of 4 elements and every block's span
-// of external entries starts at a multiple of 4 entries (padding: value 0,
-// column of a row of the block), so the block's operands are staged into LDS
-// with 16-byte loads, all of them independent:
-//  1. stage the packed inverse and the external entries (columns, values);
-//  2. G threads per row: t_i = b_i - sum over the row's external entries of
-//     value * finished value of an earlier (forward) / later (backward) stage;
-//  3. G threads per row: y_i = sum_j inv(i, j) t_j from LDS.
-// Blocks with more than ext_cap external entries read them from global memory.
-template <typename TV, typename T, typename TB, typename TY, int G, bool FWD>
-__global__ __launch_bounds__(BLOCK) void k_ilu_stage_dense(DenseTri<TV> F, int blk0, int inv_cap, int ext_cap,
-                                                           const TB* b, TY* y) {
+// LDS of one dense stage workgroup: packed inverse(s) (tv bytes), one
+// partial sum per external chunk and the right-hand side(s) (ta bytes)
+__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int part_cap, int tv, int ta, int mode) {
+  const int ninv = mode == DS_BOTH ? 2 : 1;
+  return (size_t)tv * ninv * inv_cap + (size_t)ta * (part_cap + 256 * ninv);
+}
+
+// One stage, one block per workgroup of NT threads. The block's external
+// entries are chunks of 4 (every row's list padded with value-0 entries), so
+// thread k reads chunks k, k + NT, ... as 16-byte column and value vectors,
+// consecutive threads at consecutive addresses, with no per-row pointer in
+// the way; the packed inverse(s) (starting at multiples of 4 elements) load
+// as 16-byte units at the same time:
+//  1. issue the first RI inverse units and RC chunks of every thread, then
+//     the gathers yin[col] of its chunks; chunk partial sums go to LDS;
+//  2. per row: t_i = b_i - (sum of the row's chunk partials in order);
+//  3. G threads per row: y_i = sum_j inv(i, j) t_j from LDS (MODE 2 then
+//     y_i = sum_j U_inv(i, j) t'_j).
+// The order of every sum is fixed, so results do not depend on scheduling.
+template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE>
+__global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV* __restrict__ uinv, int blk0,
+                                                        int inv_cap, int part_cap, const TB* b, const TY* yin,
+                                                        TY* y) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
-  using V = typename Vec16<TV>::type;
-  constexpr int VN = Vec16<TV>::n;
+  typedef TV V4 __attribute__((ext_vector_type(4)));
+  constexpr int VN = 16 / sizeof(TV);
+  constexpr bool FWD = MODE != DS_BWD;
+  constexpr int NINV = MODE == DS_BOTH ? 2 : 1;
+  constexpr int RI = sizeof(TV) == 4 ? 3 : 5;                 // inverse units per thread, first round
+  constexpr int RC = sizeof(TV) + sizeof(TY) <= 8 ? 5 : 3;    // chunks per thread, first round
   TV* Ms = (TV*)dsm;
-  TV* ev = Ms + inv_cap;
-  T* t = (T*)(ev + ext_cap);  // inv_cap, ext_cap: multiples of 4, so 16-byte aligned
-  int32_t* ec = (int32_t*)(t + 256);
-  const int blk = blk0 + blockIdx.x;
+  TV* Us = Ms + inv_cap;
+  T* part = (T*)(Ms + NINV * inv_cap);
+  T* t = part + part_cap;
+  T* t2 = t + 256;
   const int tid = threadIdx.x;
-  const int r0 = F.blk_ptr[blk], nr = F.blk_ptr[blk + 1] - r0;
-  const TV* M = F.inv + F.inv_off[blk];
-  const int msz = nr * (nr + 1) / 2;
-  const int64_t e0 = F.xptr[r0];
-  const int ne = (int)(F.xptr[r0 + nr] - e0);  // a multiple of 4
-  const bool staged = ne <= ext_cap;
-  {
-    const int nv = (msz + VN - 1) / VN;
-    for (int k = tid; k < nv; k += BLOCK) ((V*)Ms)[k] = __builtin_nontemporal_load((const V*)M + k);
-    if (staged) {
-      for (int k = tid; k < ne / VN; k += BLOCK) ((V*)ev)[k] = __builtin_nontemporal_load((const V*)(F.xv + e0) + k);
-      for (int k = tid; k < ne / 4; k += BLOCK)
-        ((i32x4*)ec)[k] = __builtin_nontemporal_load((const i32x4*)(F.xci + e0) + k);
+  const int blk = blk0 + blockIdx.x;
+  const i32x4 d = F.desc[blk];
+  const int r0 = d.x, nr = d.y, c0 = d.z, nc = d.w;
+  const int64_t mo = F.inv_off[blk];
+  const int nv = (nr * (nr + 1) / 2 + VN - 1) / VN;
+  // blocks with more chunks than LDS holds partial sums for sum per row
+  const bool chunked = nc <= part_cap;
+  const i32x4* isrc = (const i32x4*)(F.inv + mo);
+  const i32x4* usrc = MODE == DS_BOTH ? (const i32x4*)(uinv + mo) : nullptr;
+  const i32x4* csrc = (const i32x4*)F.xci + c0;
+  const V4* vsrc = (const V4*)F.xv + c0;
+  i32x4 iv[RI], uv[RI];
+#pragma unroll
+  for (int u = 0; u < RI; ++u) {
+    const int k = u * NT + tid;
+    if (k < nv) {
+      iv[u] = __builtin_nontemporal_load(isrc + k);
+      if (MODE == DS_BOTH) uv[u] = __builtin_nontemporal_load(usrc + k);
     }
+  }
+  i32x4 cv[RC];
+  V4 vv[RC];
+#pragma unroll
+  for (int u = 0; u < RC; ++u) {
+    const int c = u * NT + tid;
+    if (chunked && c < nc) {
+      cv[u] = __builtin_nontemporal_load(csrc + c);
+      vv[u] = __builtin_nontemporal_load(vsrc + c);
+    }
+  }
+  int rg0 = 0;
+  T b0 = 0;
+  if (tid < nr) {
+    rg0 = F.rng[r0 + tid];
+    b0 = (T)b[r0 + tid];
+  }
+#pragma unroll
+  for (int u = 0; u < RI; ++u) {
+    const int k = u * NT + tid;
+    if (k < nv) {
+      ((i32x4*)Ms)[k] = iv[u];
+      if (MODE == DS_BOTH) ((i32x4*)Us)[k] = uv[u];
+    }
+  }
+  TY yv[RC][4];
+#pragma unroll
+  for (int u = 0; u < RC; ++u)
+    if (chunked && u * NT + tid < nc) {
+      yv[u][0] = yin[cv[u].x];
+      yv[u][1] = yin[cv[u].y];
+      yv[u][2] = yin[cv[u].z];
+      yv[u][3] = yin[cv[u].w];
+    }
+#pragma unroll
+  for (int u = 0; u < RC; ++u) {
+    const int c = u * NT + tid;
+    if (chunked && c < nc) {
+      T s = (T)vv[u].x * (T)yv[u][0];
+      s += (T)vv[u].y * (T)yv[u][1];
+      s += (T)vv[u].z * (T)yv[u][2];
+      s += (T)vv[u].w * (T)yv[u][3];
+      part[c] = s;
+    }
+  }
+  // blocks larger than one round: two units / chunks per thread in flight
+  for (int k = RI * NT + tid; k < nv; k += 2 * NT) {
+    const bool two = k + NT < nv;
+    const i32x4 a0 = __builtin_nontemporal_load(isrc + k);
+    const i32x4 a1 = two ? __builtin_nontemporal_load(isrc + k + NT) : a0;
+    ((i32x4*)Ms)[k] = a0;
+    if (two) ((i32x4*)Ms)[k + NT] = a1;
+    if (MODE == DS_BOTH) {
+      const i32x4 u0 = __builtin_nontemporal_load(usrc + k);
+      const i32x4 u1 = two ? __builtin_nontemporal_load(usrc + k + NT) : u0;
+      ((i32x4*)Us)[k] = u0;
+      if (two) ((i32x4*)Us)[k + NT] = u1;
+    }
+  }
+  for (int c = RC * NT + tid; chunked && c < nc; c += 2 * NT) {
+    const int c1 = c + NT < nc ? c + NT : c;
+    const i32x4 ca = __builtin_nontemporal_load(csrc + c), cb = __builtin_nontemporal_load(csrc + c1);
+    const V4 va = __builtin_nontemporal_load(vsrc + c), vb = __builtin_nontemporal_load(vsrc + c1);
+    const TY ya0 = yin[ca.x], ya1 = yin[ca.y], ya2 = yin[ca.z], ya3 = yin[ca.w];
+    const TY yb0 = yin[cb.x], yb1 = yin[cb.y], yb2 = yin[cb.z], yb3 = yin[cb.w];
+    T s = (T)va.x * (T)ya0;
+    s += (T)va.y * (T)ya1;
+    s += (T)va.z * (T)ya2;
+    s += (T)va.w * (T)ya3;
+    part[c] = s;
+    s = (T)vb.x * (T)yb0;
+    s += (T)vb.y * (T)yb1;
+    s += (T)vb.z * (T)yb2;
+    s += (T)vb.w * (T)yb3;
+    part[c1] = s;
+  }
+  __syncthreads();
+  for (int i = tid; i < nr; i += NT) {
+    const uint32_t rg = (uint32_t)(i == tid ? rg0 : F.rng[r0 + i]);
+    const T bi = i == tid ? b0 : (T)b[r0 + i];
+    const int cs = (int)(rg & 0xffffu), cn = (int)(rg >> 16);
+    T s = 0;
+    if (chunked) {
+      T s1 = 0;
+      int k = 0;
+      for (; k + 1 < cn; k += 2) {
+        s += part[cs + k];
+        s1 += part[cs + k + 1];
+      }
+      if (k < cn) s += part[cs + k];
+      s += s1;
+    } else {
+      for (int k = cs; k < cs + cn; ++k) {
+        const i32x4 cc = csrc[k];
+        const V4 v = vsrc[k];
+        T q = (T)v.x * (T)yin[cc.x];
+        q += (T)v.y * (T)yin[cc.y];
+        q += (T)v.z * (T)yin[cc.z];
+        q += (T)v.w * (T)yin[cc.w];
+        s += q;
+      }
+    }
+    t[i] = bi - s;
   }
   __syncthreads();
   const int grp = tid / G, gl = tid % G;
-  for (int i = grp; i < nr; i += BLOCK / G) {
-    const int r = r0 + i;
-    const int p0 = (int)(F.xptr[r] - e0), p1 = (int)(F.xptr[r + 1] - e0);
-    T acc = 0;
-    for (int p = p0 + gl; p < p1; p += 4 * G) {
-      TV v[4];
-      TY xg[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int q0 = p + u * G;
-        const int q = q0 < p1 ? q0 : p1 - 1;
-        const int32_t c = staged ? ec[q] : F.xci[e0 + q];
-        v[u] = q0 < p1 ? (staged ? ev[q] : F.xv[e0 + q]) : (TV)0;
-        xg[u] = y[c];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc += (T)v[u] * (T)xg[u];
-    }
+  for (int i = grp; i < nr; i += NT / G) {
+    T acc = FWD ? lower_row_dot<TV, T, G>(Ms, t, nr, i, gl) : upper_row_dot<TV, T, G>(Ms, t, nr, i, gl);
     acc = group_reduce<G>(acc);
-    if (gl == 0) t[i] = (T)b[r] - acc;
+    if (gl == 0) {
+      if (MODE == DS_BOTH) t2[i] = acc;
+      else y[r0 + i] = (TY)acc;
+    }
   }
-  __syncthreads();
-  for (int i = grp; i < nr; i += BLOCK / G) {
-    T acc = 0;
-    if (FWD) {  // columns j <= i; column j starts at j nr - j (j - 1) / 2
-      for (int j = gl; j <= i; j += G) acc += (T)Ms[j * nr - j * (j - 1) / 2 + i - j] * t[j];
-    } else {    // columns j >= i; column j starts at j (j + 1) / 2
-      for (int j = i + gl; j < nr; j += G) acc += (T)Ms[j * (j + 1) / 2 + i] * t[j];
+  if (MODE == DS_BOTH) {
+    __syncthreads();
+    for (int i = grp; i < nr; i += NT / G) {
+      T acc = upper_row_dot<TV, T, G>(Us, t2, nr, i, gl);
+      acc = group_reduce<G>(acc);
+      if (gl == 0) y[r0 + i] = (TY)acc;
     }
-    acc = group_reduce<G>(acc);
-    if (gl == 0) y[r0 + i] = (TY)acc;
   }
 }
 

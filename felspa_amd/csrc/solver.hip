@@ -295,59 +295,70 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     linv32.alloc(4 * std::max<int64_t>(4, inv_total));
     uinv32.alloc(4 * std::max<int64_t>(4, inv_total));
     // external entries, L part [0, o_lint) and U part [o_uext, len) of every
-    // row, block after block; every block's span padded to a multiple of 4
-    // (padding: position -1 = value 0, column = the block's first row)
-    std::vector<int64_t> xl(n + 1, 0), xu(n + 1, 0);
-    int64_t ql = 0, qu = 0;
-    xl_max = xu_max = 0;
-    for (int b = 0; b < sched.n_blocks(); ++b) {
-      const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
-      const int64_t sl = ql, su = qu;
-      for (int32_t r = r0; r < r1; ++r) {
-        xl[r] = ql;
-        xu[r] = qu;
-        ql += ol[r];
-        qu += F.rp[r + 1] - F.rp[r] - ou[r];
-      }
-      ql = (ql + 3) & ~int64_t(3);
-      qu = (qu + 3) & ~int64_t(3);
-      xl_max = std::max<int>(xl_max, (int)(ql - sl));
-      xu_max = std::max<int>(xu_max, (int)(qu - su));
-    }
-    xl[n] = ql;
-    xu[n] = qu;
-    n_xl = ql;
-    n_xu = qu;
-    std::vector<int32_t> lci(std::max<int64_t>(4, n_xl)), uci(std::max<int64_t>(4, n_xu));
-    std::vector<int64_t> lpo2(std::max<int64_t>(4, n_xl), -1), upo2(std::max<int64_t>(4, n_xu), -1);
-    for (int b = 0; b < sched.n_blocks(); ++b) {
-      const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
-      for (int64_t q = xl[r0]; q < xl[r1]; ++q) lci[q] = r0;
-      for (int64_t q = xu[r0]; q < xu[r1]; ++q) uci[q] = r0;
-      for (int32_t r = r0; r < r1; ++r) {
-        const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
-        int64_t q = xl[r];
-        for (int32_t p = 0; p < ol[r]; ++p, ++q) {
-          lci[q] = F.ci[base + p];
-          lpo2[q] = base + p;
+    // row, in chunks of 4: every row's list padded to a multiple of 4
+    // (padding: position -1 = value 0, column = the row's first external
+    // column, a finished value that is read and multiplied by 0), rows of a
+    // block after each other. Per block {first row, rows, first chunk,
+    // chunks}; per row its first chunk relative to the block (low 16 bits)
+    // and its chunk count (high 16 bits).
+    auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& dci, DevBuf& dpos, DevBuf& dv64, DevBuf& dv32,
+                   int64_t& n_ent, int& c_max) {
+      const int nb = sched.n_blocks();
+      std::vector<int32_t> dsc(4 * (size_t)nb), rg(n);
+      int64_t q = 0;  // chunks so far
+      c_max = 0;
+      for (int b = 0; b < nb; ++b) {
+        const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+        const int64_t qb = q;
+        for (int32_t r = r0; r < r1; ++r) {
+          const int64_t cnt = lower ? ol[r] : F.rp[r + 1] - F.rp[r] - ou[r];
+          const int64_t nc = (cnt + 3) / 4;
+          if (q - qb > 65535 || nc > 65535) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external entries");
+          rg[r] = (int32_t)((q - qb) | (nc << 16));
+          q += nc;
         }
-        q = xu[r];
-        for (int64_t p = ou[r]; p < len; ++p, ++q) {
-          uci[q] = F.ci[base + p];
-          upo2[q] = base + p;
-        }
+        if (q > INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external entries");
+        dsc[4 * b] = r0;
+        dsc[4 * b + 1] = r1 - r0;
+        dsc[4 * b + 2] = (int32_t)qb;
+        dsc[4 * b + 3] = (int32_t)(q - qb);
+        c_max = std::max<int>(c_max, (int)(q - qb));
       }
+      n_ent = 4 * q;
+      std::vector<int32_t> cc(std::max<int64_t>(4, n_ent));
+      std::vector<int64_t> po(std::max<int64_t>(4, n_ent), -1);
+      int64_t e = 0;
+      for (int b = 0; b < nb; ++b)
+        for (int32_t r = sched.blk_ptr[b]; r < sched.blk_ptr[b + 1]; ++r) {
+          const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
+          const int64_t p0 = lower ? 0 : ou[r], p1 = lower ? ol[r] : len;
+          const int64_t e0 = e;
+          for (int64_t p = p0; p < p1; ++p, ++e) {
+            cc[e] = F.ci[base + p];
+            po[e] = base + p;
+          }
+          for (; (e - e0) % 4; ++e) cc[e] = cc[e0];
+        }
+      desc.upload(dsc.data(), 4 * dsc.size());
+      rng.upload(rg.data(), 4 * rg.size());
+      dci.upload(cc.data(), 4 * cc.size());
+      dpos.upload(po.data(), 8 * po.size());
+      dv64.alloc(8 * cc.size());
+      dv32.alloc(4 * cc.size());
+    };
+    n_ext = sum_lext + sum_uext;
+    ext(true, xldesc, xlrng, xlci, xlpos, xlv64, xlv32, n_xl, xl_max);
+    ext(false, xudesc, xurng, xuci, xupos, xuv64, xuv32, n_xu, xu_max);
+    // the last stage has no external U entries when no two blocks of a stage
+    // couple: its backward solve is then in-block only and runs in the same
+    // launch as its forward solve
+    const int ns = sched.n_stages();
+    fuse_last = false;
+    if (ns > 0) {
+      int64_t cnt = 0;
+      for (int32_t r = sched.blk_ptr[sched.stage_ptr[ns - 1]]; r < n; ++r) cnt += F.rp[r + 1] - F.rp[r] - ou[r];
+      fuse_last = cnt == 0;
     }
-    xlptr.upload(xl.data(), 8 * xl.size());
-    xuptr.upload(xu.data(), 8 * xu.size());
-    xlci.upload(lci.data(), 4 * lci.size());
-    xuci.upload(uci.data(), 4 * uci.size());
-    xlpos.upload(lpo2.data(), 8 * lpo2.size());
-    xupos.upload(upo2.data(), 8 * upo2.size());
-    xlv64.alloc(8 * lci.size());
-    xlv32.alloc(4 * lci.size());
-    xuv64.alloc(8 * uci.size());
-    xuv32.alloc(4 * uci.size());
   }
   host_pattern = std::move(F);
 }
@@ -463,40 +474,56 @@ static size_t dense_wg_lds_limit() {
   return lim;
 }
 
-template <typename TV, typename T, typename TB, typename TY, int G, bool FWD>
-static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
+template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE>
+static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
   const int nb = b1 - b0;
   if (nb <= 0) return;
   const bool d = std::is_same<TV, double>::value;
+  const bool fwd = MODE != DS_BWD;
   DenseTri<TV> D;
-  D.blk_ptr = F.blk_ptr.as<int32_t>();
-  D.xptr = FWD ? F.xlptr.as<int64_t>() : F.xuptr.as<int64_t>();
-  D.xci = FWD ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
-  D.xv = FWD ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
+  D.desc = fwd ? F.xldesc.as<i32x4>() : F.xudesc.as<i32x4>();
+  D.rng = fwd ? F.xlrng.as<int32_t>() : F.xurng.as<int32_t>();
+  D.xci = fwd ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
+  D.xv = fwd ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
   D.inv_off = F.inv_off.as<int64_t>();
-  D.inv = FWD ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
-  // external entries staged in LDS up to what the workgroup's LDS allows
-  const size_t lim = dense_wg_lds_limit();
-  const size_t fixed = dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T));
-  const int want = FWD ? F.xl_max : F.xu_max;
-  int ext_cap = lim > fixed + 64 ? (int)std::min<size_t>(want, (lim - fixed - 16) / (sizeof(TV) + 4)) : 0;
-  ext_cap &= ~3;
-  const size_t lds = dense_lds_bytes(F.inv_cap, ext_cap, sizeof(TV), sizeof(T));
-  auto fn = &k_ilu_stage_dense<TV, T, TB, TY, G, FWD>;
+  D.inv = fwd ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
+  const TV* uinv = MODE == DS_BOTH ? (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p) : nullptr;
+  // partial sums of every chunk of the largest block, or of what LDS holds
+  // (larger blocks sum per row)
+  const size_t room = dense_wg_lds_limit() - dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T), MODE);
+  const int part_cap = (int)std::min<size_t>(((fwd ? F.xl_max : F.xu_max) + 3) & ~3, (room / sizeof(T)) & ~(size_t)3);
+  const size_t lds = dense_lds_bytes(F.inv_cap, part_cap, sizeof(TV), sizeof(T), MODE);
+  auto fn = &k_ilu_stage_dense<TV, T, TB, TY, NT, G, MODE>;
   if (lds > 64 * 1024)
     hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
               "hipFuncSetAttribute");
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(BLOCK), lds, st, D, b0, F.inv_cap, ext_cap, b, y);
+  hipLaunchKernelGGL(fn, dim3(nb), dim3(NT), lds, st, D, uinv, b0, F.inv_cap, part_cap, b, yin, y);
 }
 
-template <typename TV, typename T, typename TB, typename TY, bool FWD>
-static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, hipStream_t st) {
-  // threads per row: the block's rows fill the 256-thread workgroup
+template <typename TV, typename T, typename TB, typename TY, int MODE>
+static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+  // 128 threads for blocks of up to 64 rows (more workgroups resident per
+  // CU), 256 above; G threads per row in the in-block product
   const int mr = F.sched.max_block_rows;
-  if (mr <= 32) launch_dense<TV, T, TB, TY, 8, FWD>(F, b0, b1, b, y, st);
-  else if (mr <= 64) launch_dense<TV, T, TB, TY, 4, FWD>(F, b0, b1, b, y, st);
-  else if (mr <= 128) launch_dense<TV, T, TB, TY, 2, FWD>(F, b0, b1, b, y, st);
-  else launch_dense<TV, T, TB, TY, 1, FWD>(F, b0, b1, b, y, st);
+  if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 128) launch_dense<TV, T, TB, TY, 256, 2, MODE>(F, b0, b1, b, yin, y, st);
+  else launch_dense<TV, T, TB, TY, 256, 1, MODE>(F, b0, b1, b, yin, y, st);
+}
+
+template <typename TV, typename T, typename TB>
+static void apply_dense(const IluDev& F, bool fuse, const TB* b, T* tmp, T* x, hipStream_t st) {
+  const Schedule& sc = F.sched;
+  const int ns = sc.n_stages();
+  const int nf = fuse ? ns - 1 : ns;  // stages with a forward launch of their own
+  for (int s = 0; s < nf; ++s)
+    launch_dense_g<TV, T, TB, T, DS_FWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], b, tmp, tmp, st);
+  if constexpr (sizeof(TV) == 4) {
+    if (fuse) launch_dense_g<TV, T, TB, T, DS_BOTH>(F, sc.stage_ptr[ns - 1], sc.stage_ptr[ns], b, tmp, x, st);
+  }
+  for (int s = nf - 1; s >= 0; --s)
+    launch_dense_g<TV, T, T, T, DS_BWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], tmp, x, x, st);
 }
 
 // x = (LU)^{-1} b with TV factor values and T vectors / arithmetic: forward
@@ -504,17 +531,19 @@ static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, TY* y, 
 template <typename TV, typename T, typename TB>
 void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
   const int ns = sched.n_stages();
-  // the dense kernels need a block's packed triangle in LDS
-  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(TV), sizeof(T)) + 64 <= dense_wg_lds_limit()) {
-    for (int s = 0; s < ns; ++s)
-      launch_dense_g<TV, T, TB, T, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
-    for (int s = ns - 1; s >= 0; --s)
-      launch_dense_g<TV, T, T, T, false>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], tmp, x, st);
-    cnt.launches += 2 * ns;
-    // external entries (value + column), both packed triangles, the row
-    // pointers of the external spans, b read, y written, gathered values
-    const double ext = (double)(n_xl + n_xu);
-    cnt.bytes += ext * (sizeof(TV) + 4) + 2.0 * inv_total * sizeof(TV) + 16.0 * n + 4.0 * n * sizeof(T);
+  // the dense kernels keep a block's packed triangle (the fused last stage
+  // both of them) and its chunk partial sums in LDS
+  const size_t lim = dense_wg_lds_limit();
+  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
+    // (FP32 factors only: with FP64 ones both triangles' registers cost more
+    // occupancy than the launch saves)
+    const int pc = (std::max(xl_max, xu_max) + 3) & ~3;
+    const bool fuse = sizeof(TV) == 4 && fuse_last && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_BOTH) <= lim;
+    apply_dense<TV, T, TB>(*this, fuse, b, tmp, x, st);
+    cnt.launches += fuse ? 2 * ns - 1 : 2 * ns;
+    // external entries (value + column), both packed triangles, the per-row
+    // chunk ranges, b read and y written in both sweeps
+    cnt.bytes += (double)n_ext * (sizeof(TV) + 4) + 2.0 * inv_total * sizeof(TV) + 8.0 * n + 4.0 * n * sizeof(T);
     return;
   }
   if (!std::is_same<TV, T>::value)

@@ -75,10 +75,12 @@ struct IluDev {
   bool dense = true;
   int64_t inv_total = 0;
   int inv_cap = 0;                 // largest packed triangle of a block
-  int64_t n_xl = 0, n_xu = 0;      // external entries of L / U
-  int xl_max = 0, xu_max = 0;      // largest per-block count of external L / U entries
+  int64_t n_ext = 0;               // external entries of L and U
+  int64_t n_xl = 0, n_xu = 0;      // external entries of L / U, padded to chunks of 4 per row
+  int xl_max = 0, xu_max = 0;      // largest per-block count of chunks of external L / U entries
+  bool fuse_last = false;          // last stage: forward and backward in one launch
   DevBuf inv_off, linv64, uinv64, linv32, uinv32;
-  DevBuf xlptr, xlci, xlpos, xlv64, xlv32, xuptr, xuci, xupos, xuv64, xuv32;
+  DevBuf xldesc, xlrng, xlci, xlpos, xlv64, xlv32, xudesc, xurng, xuci, xupos, xuv64, xuv32;
   void gather_inblock(hipStream_t st);
   void build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max);
   template <typename T>
