@@ -678,6 +678,13 @@ struct DenseTri {
   const T* inv;           // packed inverse of the in-block factor
 };
 
+// Chunks of external entries per thread loaded with the inverse in the
+// dense stage kernel; part_cap >= dense_rc * threads or >= every block's chunks
+template <typename TV, typename TY>
+__host__ __device__ constexpr int dense_rc() {
+  return sizeof(TV) + sizeof(TY) <= 8 ? 5 : 3;
+}
+
 // Lane gl's share of row i of a packed inverse times t, columns gl, gl + G,
 // ...: lower (columns j <= i, column j starting at j n - j (j - 1) / 2) and
 // upper (columns j >= i, column j starting at j (j + 1) / 2); four
@@ -744,8 +751,8 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   constexpr int VN = 16 / sizeof(TV);
   constexpr bool FWD = MODE != DS_BWD;
   constexpr int NINV = MODE == DS_BOTH ? 2 : 1;
-  constexpr int RI = sizeof(TV) == 4 ? 3 : 5;                 // inverse units per thread, first round
-  constexpr int RC = sizeof(TV) + sizeof(TY) <= 8 ? 5 : 3;    // chunks per thread, first round
+  constexpr int RI = sizeof(TV) == 4 ? 3 : 5;  // inverse units per thread, first round
+  constexpr int RC = dense_rc<TV, TY>();         // chunks per thread, first round
   TV* Ms = (TV*)dsm;
   TV* Us = Ms + inv_cap;
   T* part = (T*)(Ms + NINV * inv_cap);
@@ -757,8 +764,6 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   const int r0 = d.x, nr = d.y, c0 = d.z, nc = d.w;
   const int64_t mo = F.inv_off[blk];
   const int nv = (nr * (nr + 1) / 2 + VN - 1) / VN;
-  // blocks with more chunks than LDS holds partial sums for sum per row
-  const bool chunked = nc <= part_cap;
   const i32x4* isrc = (const i32x4*)(F.inv + mo);
   const i32x4* usrc = MODE == DS_BOTH ? (const i32x4*)(uinv + mo) : nullptr;
   const i32x4* csrc = (const i32x4*)F.xci + c0;
@@ -772,21 +777,26 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (MODE == DS_BOTH) uv[u] = __builtin_nontemporal_load(usrc + k);
     }
   }
+  // chunks in passes of part_cap (>= RC NT) partial sums; the first RC NT
+  // chunks load with the inverse
   i32x4 cv[RC];
   V4 vv[RC];
 #pragma unroll
   for (int u = 0; u < RC; ++u) {
     const int c = u * NT + tid;
-    if (chunked && c < nc) {
+    if (c < nc) {
       cv[u] = __builtin_nontemporal_load(csrc + c);
       vv[u] = __builtin_nontemporal_load(vsrc + c);
     }
   }
-  int rg0 = 0;
-  T b0 = 0;
+  // one row per thread (nr <= NT)
+  int cs = 0, cn = 0;
+  T bi = 0;
   if (tid < nr) {
-    rg0 = F.rng[r0 + tid];
-    b0 = (T)b[r0 + tid];
+    const uint32_t rg = (uint32_t)F.rng[r0 + tid];
+    cs = (int)(rg & 0xffffu);
+    cn = (int)(rg >> 16);
+    bi = (T)b[r0 + tid];
   }
 #pragma unroll
   for (int u = 0; u < RI; ++u) {
@@ -799,7 +809,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   TY yv[RC][4];
 #pragma unroll
   for (int u = 0; u < RC; ++u)
-    if (chunked && u * NT + tid < nc) {
+    if (u * NT + tid < nc) {
       yv[u][0] = yin[cv[u].x];
       yv[u][1] = yin[cv[u].y];
       yv[u][2] = yin[cv[u].z];
@@ -808,7 +818,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
 #pragma unroll
   for (int u = 0; u < RC; ++u) {
     const int c = u * NT + tid;
-    if (chunked && c < nc) {
+    if (c < nc) {
       T s = (T)vv[u].x * (T)yv[u][0];
       s += (T)vv[u].y * (T)yv[u][1];
       s += (T)vv[u].z * (T)yv[u][2];
@@ -816,7 +826,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       part[c] = s;
     }
   }
-  // blocks larger than one round: two units / chunks per thread in flight
+  // the rest of the inverse: two units per thread in flight
   for (int k = RI * NT + tid; k < nv; k += 2 * NT) {
     const bool two = k + NT < nv;
     const i32x4 a0 = __builtin_nontemporal_load(isrc + k);
@@ -830,51 +840,38 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (two) ((i32x4*)Us)[k + NT] = u1;
     }
   }
-  for (int c = RC * NT + tid; chunked && c < nc; c += 2 * NT) {
-    const int c1 = c + NT < nc ? c + NT : c;
-    const i32x4 ca = __builtin_nontemporal_load(csrc + c), cb = __builtin_nontemporal_load(csrc + c1);
-    const V4 va = __builtin_nontemporal_load(vsrc + c), vb = __builtin_nontemporal_load(vsrc + c1);
-    const TY ya0 = yin[ca.x], ya1 = yin[ca.y], ya2 = yin[ca.z], ya3 = yin[ca.w];
-    const TY yb0 = yin[cb.x], yb1 = yin[cb.y], yb2 = yin[cb.z], yb3 = yin[cb.w];
-    T s = (T)va.x * (T)ya0;
-    s += (T)va.y * (T)ya1;
-    s += (T)va.z * (T)ya2;
-    s += (T)va.w * (T)ya3;
-    part[c] = s;
-    s = (T)vb.x * (T)yb0;
-    s += (T)vb.y * (T)yb1;
-    s += (T)vb.z * (T)yb2;
-    s += (T)vb.w * (T)yb3;
-    part[c1] = s;
-  }
-  __syncthreads();
-  for (int i = tid; i < nr; i += NT) {
-    const uint32_t rg = (uint32_t)(i == tid ? rg0 : F.rng[r0 + i]);
-    const T bi = i == tid ? b0 : (T)b[r0 + i];
-    const int cs = (int)(rg & 0xffffu), cn = (int)(rg >> 16);
-    T s = 0;
-    if (chunked) {
-      T s1 = 0;
-      int k = 0;
-      for (; k + 1 < cn; k += 2) {
-        s += part[cs + k];
-        s1 += part[cs + k + 1];
-      }
-      if (k < cn) s += part[cs + k];
-      s += s1;
-    } else {
-      for (int k = cs; k < cs + cn; ++k) {
-        const i32x4 cc = csrc[k];
-        const V4 v = vsrc[k];
-        T q = (T)v.x * (T)yin[cc.x];
-        q += (T)v.y * (T)yin[cc.y];
-        q += (T)v.z * (T)yin[cc.z];
-        q += (T)v.w * (T)yin[cc.w];
-        s += q;
-      }
+  // row sums in chunk order, two partial sums by the parity of the chunk
+  // within its row (the same split whatever the passes)
+  T s0 = 0, s1 = 0;
+  for (int cb = 0; cb < nc; cb += part_cap) {
+    const int ce = nc < cb + part_cap ? nc : cb + part_cap;
+    // chunks [cb, ce) not yet summed into part: two per thread in flight
+    for (int c = (cb == 0 ? RC * NT : cb) + tid; c < ce; c += 2 * NT) {
+      const int c1 = c + NT < ce ? c + NT : c;
+      const i32x4 ca = __builtin_nontemporal_load(csrc + c), cc = __builtin_nontemporal_load(csrc + c1);
+      const V4 va = __builtin_nontemporal_load(vsrc + c), vb = __builtin_nontemporal_load(vsrc + c1);
+      const TY ya0 = yin[ca.x], ya1 = yin[ca.y], ya2 = yin[ca.z], ya3 = yin[ca.w];
+      const TY yb0 = yin[cc.x], yb1 = yin[cc.y], yb2 = yin[cc.z], yb3 = yin[cc.w];
+      T s = (T)va.x * (T)ya0;
+      s += (T)va.y * (T)ya1;
+      s += (T)va.z * (T)ya2;
+      s += (T)va.w * (T)ya3;
+      part[c - cb] = s;
+      s = (T)vb.x * (T)yb0;
+      s += (T)vb.y * (T)yb1;
+      s += (T)vb.z * (T)yb2;
+      s += (T)vb.w * (T)yb3;
+      part[c1 - cb] = s;
     }
-    t[i] = bi - s;
+    __syncthreads();
+    const int lo = cs > cb ? cs : cb, hi = cs + cn < ce ? cs + cn : ce;
+    for (int k = lo; k < hi; ++k) {
+      if ((k - cs) & 1) s1 += part[k - cb];
+      else s0 += part[k - cb];
+    }
+    if (ce < nc) __syncthreads();  // part is overwritten by the next pass
   }
+  if (tid < nr) t[tid] = bi - (s0 + s1);
   __syncthreads();
   const int grp = tid / G, gl = tid % G;
   for (int i = grp; i < nr; i += NT / G) {
@@ -941,69 +938,158 @@ static __global__ __launch_bounds__(BLOCK) void k_ilu_block_inverse(IluView<doub
 
 // ---------------------------------------------------------------------------
 // Numeric ILU (include/felspa/linear_algebra/src/ilu.implement.h:438-541):
-// each wave factorises the rows of one block in order. The row is held in LDS
-// (sorted columns + working values); the source matrix row (columns <
-// col_limit) is scattered in, then for each L column k in ascending order
-// m = w_k / u_kk and w_j -= m u_kj for the columns j > k of row k that row i
-// stores. FP64 throughout; err[0] is set on a zero or non-finite pivot.
-static __global__ __launch_bounds__(WAVE) void k_ilu_numeric_stage(
-  IluView<double> F, int blk0, int blk1, const int64_t* __restrict__ srp, const int32_t* __restrict__ sci,
-  const double* __restrict__ sv, int64_t col_limit, int max_len, int* __restrict__ err) {
-  extern __shared__ char smem[];
-  int32_t* lc = (int32_t*)smem;
-  double* w = (double*)(smem + ((4 * (size_t)max_len + 15) & ~(size_t)15));
-  const int lane = threadIdx.x;
-  for (int blk = blk0 + blockIdx.x; blk < blk1; blk += gridDim.x) {
-    const int r0 = F.blk_ptr[blk], r1 = F.blk_ptr[blk + 1];
-    for (int r = r0; r < r1; ++r) {
-      const int64_t base = F.rp[r];
-      const int len = (int)(F.rp[r + 1] - base);
-      for (int t = lane; t < len; t += WAVE) {
-        lc[t] = F.ci[base + t];
-        w[t] = 0.0;
-      }
-      wave_sync();
-      for (int64_t p = srp[r] + lane; p < srp[r + 1]; p += WAVE) {
-        const int32_t c = sci[p];
-        if (c >= col_limit) continue;
-        int lo = 0, hi = len;
-        while (lo < hi) {
-          int mid = (lo + hi) >> 1;
-          if (lc[mid] < c) lo = mid + 1; else hi = mid;
+// each workgroup of NT threads factorises the rows of one block in order.
+// The row is held in LDS (columns, working values, and a hash map column ->
+// position); the source matrix row (columns < col_limit) is scattered in,
+// then for each L column k in ascending order m = w_k / u_kk and
+// w_j -= m u_kj for the columns j > k of row k that row i stores. Rows k < i
+// are final, so every row k's U span and pivot are read for all L columns at
+// once, and row k+1's U part loads while row k's is applied (the barrier
+// between L columns waits for LDS only). FP64 throughout; err[0] is set on a
+// zero or non-finite pivot.
+__host__ __device__ inline int ilu_numeric_hash_size(int max_len) {
+  int h = 64;
+  while (h < 4 * max_len) h *= 2;  // load factor <= 1/4: lookups of dropped
+                                   // fill entries stay short
+  return h;
+}
+__host__ __device__ inline size_t ilu_numeric_lds_bytes(int max_len) {
+  return (size_t)max_len * (8 + 8 + 8 + 4 + 4) + (size_t)ilu_numeric_hash_size(max_len) * 8;
+}
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+// multiplicative (Fibonacci) hashing: the top bits of c * 2^32 / phi
+__device__ __forceinline__ uint32_t ilu_hash(int32_t c, uint32_t mask) {
+  return ((uint32_t)c * 2654435769u) >> (32 - __builtin_popcount(mask));
+}
+__device__ __forceinline__ int ilu_find(const int32_t* keys, const int32_t* vals, uint32_t mask, int32_t c) {
+  for (uint32_t h = ilu_hash(c, mask);; h = (h + 1) & mask) {
+    const int32_t k = keys[h];
+    if (k == c) return vals[h];
+    if (k < 0) return -1;
+  }
+}
+template <int NT, int M>
+__global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int blk0, int blk1,
+                                                          const int64_t* __restrict__ srp,
+                                                          const int32_t* __restrict__ sci,
+                                                          const double* __restrict__ sv, int64_t col_limit,
+                                                          int max_len, int* __restrict__ err) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int hs = ilu_numeric_hash_size(max_len);
+  const uint32_t mask = (uint32_t)hs - 1;
+  double* w = (double*)smem;                 // working row
+  int64_t* qs = (int64_t*)(w + max_len);     // per L column k: first U entry of row k
+  double* uk = (double*)(qs + max_len);      // per L column k: pivot u_kk
+  int32_t* lc = (int32_t*)(uk + max_len);    // columns of the row
+  int32_t* qn = lc + max_len;                // per L column k: U entries of row k
+  int32_t* keys = qn + max_len;              // hash map column -> position
+  int32_t* vals = keys + hs;
+  const int tid = threadIdx.x;
+  const int blk = blk0 + blockIdx.x;
+  if (blk >= blk1) return;
+  const int r0 = F.blk_ptr[blk], r1 = F.blk_ptr[blk + 1];
+  for (int r = r0; r < r1; ++r) {
+    const int64_t base = F.rp[r];
+    const int len = (int)(F.rp[r + 1] - base);
+    const int nl = F.o_diag[r];
+    for (int h = tid; h < hs; h += NT) keys[h] = -1;
+    lds_barrier();
+    for (int t = tid; t < len; t += NT) {
+      const int32_t c = F.ci[base + t];
+      lc[t] = c;
+      w[t] = 0.0;
+      for (uint32_t h = ilu_hash(c, mask);; h = (h + 1) & mask)
+        if (atomicCAS(&keys[h], -1, c) == -1) {
+          vals[h] = t;
+          break;
         }
-        if (lo < len && lc[lo] == c) w[lo] = sv[p];
+      if (t < nl) {
+        const int64_t kb = F.rp[c];
+        const int kd = F.o_diag[c];
+        qs[t] = kb + kd + 1;
+        qn[t] = (int)(F.rp[c + 1] - kb - kd - 1);
+        uk[t] = F.v[kb + kd];
       }
-      wave_sync();
-      const int nl = F.o_diag[r];
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      for (int t = 0; t < nl; ++t) {
-        const int32_t kk = lc[t];
-        const int64_t kb = F.rp[kk];
-        const int kd = F.o_diag[kk];
-        const double ukk = F.v[kb + kd];
-        const double mult = w[t] / ukk;
-        wave_sync();
-        if (lane == 0) w[t] = mult;
-        const int64_t q1 = F.rp[kk + 1];
-        for (int64_t q = kb + kd + 1 + lane; q < q1; q += WAVE) {
-          const int32_t j = F.ci[q];
-          int lo = t + 1, hi = len;
-          while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (lc[mid] < j) lo = mid + 1; else hi = mid;
-          }
-          if (lo < len && lc[lo] == j) w[lo] -= mult * F.v[q];
-        }
-        wave_sync();
-      }
-      for (int t = lane; t < len; t += WAVE) F.v[base + t] = w[t];
-      if (lane == 0) {
-        const double d = w[F.o_diag[r]];
-        if (!(d != 0.0) || !isfinite(d)) atomicExch(err, 1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      wave_sync();
     }
+    lds_barrier();
+    for (int64_t p = srp[r] + tid; p < srp[r + 1]; p += NT) {
+      const int32_t c = sci[p];
+      if (c >= col_limit) continue;
+      const int pos = ilu_find(keys, vals, mask, c);
+      if (pos >= 0) w[pos] = sv[p];
+    }
+    lds_barrier();
+    // three register sets in rotation: the U part of L column t + 3 loads
+    // while columns t + 1 and t + 2 are applied
+    int32_t ja[M], jb[M], jc[M];
+    double va[M], vb[M], vc[M];
+    // (every load is issued, from entry 0 when out of range, so the number
+    // of loads in flight is the same on every path and the waits before a
+    // set's use leave the two younger sets in flight)
+    // (nothing looks at a loaded value until it is used)
+    auto fetch = [&](int t, int32_t* j, double* v) {
+      const bool ok = t < nl;
+      const int tt = ok ? t : 0;
+      const int64_t q = qs[tt];
+      const int n = ok ? qn[tt] : 0;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        const int idx = tid + m * NT;
+        const int64_t a = idx < n ? q + idx : 0;
+        j[m] = F.ci[a];
+        v[m] = F.v[a];
+      }
+    };
+    auto apply = [&](int t, const int32_t* j, const double* v) {
+      // w[t] is final: later L columns only update positions > t
+      const double mult = w[t] / uk[t];
+      const int n = qn[t];
+      int pos[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) pos[m] = tid + m * NT < n ? ilu_find(keys, vals, mask, j[m]) : -1;
+#pragma unroll
+      for (int m = 0; m < M; ++m)
+        if (pos[m] >= 0) w[pos[m]] -= mult * v[m];
+      const int64_t q = qs[t];
+      for (int idx = M * NT + tid; idx < n; idx += NT) {  // U parts longer than M NT
+        const int p = ilu_find(keys, vals, mask, F.ci[q + idx]);
+        if (p >= 0) w[p] -= mult * F.v[q + idx];
+      }
+    };
+    fetch(0, ja, va);
+    fetch(1, jb, vb);
+    fetch(2, jc, vc);
+    int t = 0;
+    for (; t + 2 < nl; t += 3) {  // (no exit inside the body: every path
+                                  // into the loop has the same loads in flight)
+      apply(t, ja, va);
+      fetch(t + 3, ja, va);
+      lds_barrier();
+      apply(t + 1, jb, vb);
+      fetch(t + 4, jb, vb);
+      lds_barrier();
+      apply(t + 2, jc, vc);
+      fetch(t + 5, jc, vc);
+      lds_barrier();
+    }
+    if (t < nl) {
+      apply(t, ja, va);
+      lds_barrier();
+    }
+    if (t + 1 < nl) {
+      apply(t + 1, jb, vb);
+      lds_barrier();
+    }
+    for (int t = tid; t < len; t += NT) F.v[base + t] = t < nl ? w[t] / uk[t] : w[t];
+    if (tid == 0) {
+      const double d = w[nl];
+      if (!(d != 0.0) || !isfinite(d)) atomicExch(err, 1);
+    }
+    __syncthreads();  // the row's values are read by the block's later rows
   }
 }
 

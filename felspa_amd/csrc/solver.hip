@@ -205,6 +205,11 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       max_len = std::max<int>(max_len, (int)len);
     }
   }
+  // longest row of every stage: the numeric kernel's LDS per launch
+  stage_max_len.assign(sched.n_stages(), 1);
+  for (int s = 0; s < sched.n_stages(); ++s)
+    for (int32_t r = sched.blk_ptr[sched.stage_ptr[s]]; r < sched.blk_ptr[sched.stage_ptr[s + 1]]; ++r)
+      stage_max_len[s] = std::max<int>(stage_max_len[s], (int)(F.rp[r + 1] - F.rp[r]));
   for (int64_t i = 0; i < n; ++i) {
     lc[i + 1] += lc[i];
     uc[i + 1] += uc[i];
@@ -393,13 +398,25 @@ static int grid_for(int64_t n, int per_block = BLOCK, int cap = 4096) {
 void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit,
                        hipStream_t st, int* d_err, Counters& cnt) {
   IluView<double> V = view<double>();
-  const size_t lds = ((4 * (size_t)max_len + 15) & ~(size_t)15) + 8 * (size_t)max_len;
-  if (lds > 160 * 1024) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel");
+  if (ilu_numeric_lds_bytes(max_len) > 160 * 1024)
+    throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel");
   for (int s = 0; s < sched.n_stages(); ++s) {
     const int b0 = sched.stage_ptr[s], b1 = sched.stage_ptr[s + 1];
-    const int g = std::max(1, std::min(b1 - b0, 8192));
-    hipLaunchKernelGGL(k_ilu_numeric_stage, dim3(g), dim3(WAVE), lds, st, V, b0, b1, srp, sci, sv, col_limit,
-                       max_len, d_err);
+    if (b1 <= b0) continue;
+    // LDS sized by the stage's longest row; one wave per block for short
+    // rows, four for long ones
+    const int ml = stage_max_len[s];
+    const size_t lds = ilu_numeric_lds_bytes(ml);
+    const bool wide = ml > 128;
+    const void* fn = wide ? (const void*)k_ilu_numeric_stage<256, 2> : (const void*)k_ilu_numeric_stage<64, 4>;
+    if (lds > 64 * 1024)
+      hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "hipFuncSetAttribute");
+    if (wide)
+      hipLaunchKernelGGL((k_ilu_numeric_stage<256, 2>), dim3(b1 - b0), dim3(256), lds, st, V, b0, b1, srp, sci, sv,
+                         col_limit, ml, d_err);
+    else
+      hipLaunchKernelGGL((k_ilu_numeric_stage<64, 4>), dim3(b1 - b0), dim3(64), lds, st, V, b0, b1, srp, sci, sv,
+                         col_limit, ml, d_err);
     cnt.launches++;
   }
   cnt.bytes += (double)nnz * 12.0 * 2.0;
@@ -488,11 +505,18 @@ static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY*
   D.inv_off = F.inv_off.as<int64_t>();
   D.inv = fwd ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
   const TV* uinv = MODE == DS_BOTH ? (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p) : nullptr;
-  // partial sums of every chunk of the largest block, or of what LDS holds
-  // (larger blocks sum per row)
-  const size_t room = dense_wg_lds_limit() - dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T), MODE);
-  const int part_cap = (int)std::min<size_t>(((fwd ? F.xl_max : F.xu_max) + 3) & ~3, (room / sizeof(T)) & ~(size_t)3);
+  // partial sums of every chunk of the largest block when the workgroup's
+  // LDS stays within 40 KiB (several workgroups per CU), else chunks in
+  // passes of at least the first round's dense_rc * NT
+  const size_t base = dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T), MODE);
+  const int need = ((fwd ? F.xl_max : F.xu_max) + 3) & ~3;
+  const int first = dense_rc<TV, TY>() * NT;
+  const size_t lim = dense_wg_lds_limit();
+  const size_t target = std::max<size_t>(40 * 1024, base + (size_t)first * sizeof(T));
+  int part_cap = need;
+  if (base + (size_t)need * sizeof(T) > target) part_cap = std::max<int>(first, (int)(((target - base) / sizeof(T)) & ~(size_t)3));
   const size_t lds = dense_lds_bytes(F.inv_cap, part_cap, sizeof(TV), sizeof(T), MODE);
+  if (lds > lim) throw FsError(FS_ERR_UNSUPPORTED, "ILU block too large for the dense stage kernel");
   auto fn = &k_ilu_stage_dense<TV, T, TB, TY, NT, G, MODE>;
   if (lds > 64 * 1024)
     hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -533,11 +557,12 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) con
   const int ns = sched.n_stages();
   // the dense kernels keep a block's packed triangle (the fused last stage
   // both of them) and its chunk partial sums in LDS
+  // (room for the first round's partial sums of 256 threads)
   const size_t lim = dense_wg_lds_limit();
-  if (dense && dense_lds_bytes(inv_cap, 0, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
+  const int pc = 256 * dense_rc<TV, T>();
+  if (dense && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
     // (FP32 factors only: with FP64 ones both triangles' registers cost more
     // occupancy than the launch saves)
-    const int pc = (std::max(xl_max, xu_max) + 3) & ~3;
     const bool fuse = sizeof(TV) == 4 && fuse_last && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_BOTH) <= lim;
     apply_dense<TV, T, TB>(*this, fuse, b, tmp, x, st);
     cnt.launches += fuse ? 2 * ns - 1 : 2 * ns;
