@@ -676,13 +676,86 @@ struct DenseTri {
   const T* xv;            // external values, chunks of 4
   const int64_t* inv_off; // per block: start of its packed triangle(s)
   const T* inv;           // packed inverse of the in-block factor
+  const int2* split;      // per block: {S, first partial}; S > 1: its chunks
+                          // were summed by S workgroups of k_ilu_ext_partials
+  void* part;             // row partial sums of split blocks, S x rows each
 };
+
+// Blocks with more than 2 DENSE_SPLIT chunks are summed in pieces of
+// DENSE_SPLIT chunks by k_ilu_ext_partials, one workgroup per piece
+constexpr int DENSE_SPLIT = 768;
+constexpr int DENSE_SPLIT_NT = 256;
 
 // Chunks of external entries per thread loaded with the inverse in the
 // dense stage kernel; part_cap >= dense_rc * threads or >= every block's chunks
 template <typename TV, typename TY>
 __host__ __device__ constexpr int dense_rc() {
   return sizeof(TV) + sizeof(TY) <= 8 ? 5 : 3;
+}
+
+// External sums of one piece (chunks [ca, ce) of block blk, ce - ca <=
+// DENSE_SPLIT) of a block with many external entries: every row's sum over
+// its chunks inside the piece, in chunk order, to part[first + piece rows +
+// row]. One workgroup per piece, all loads in one round.
+template <typename TV, typename T, typename TY>
+__global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV> F, const i32x4* __restrict__ pieces,
+                                                                     int pc0, const TY* yin) {
+  typedef TV V4 __attribute__((ext_vector_type(4)));
+  constexpr int NT = DENSE_SPLIT_NT;
+  constexpr int R = (DENSE_SPLIT + NT - 1) / NT;
+  __shared__ T part[DENSE_SPLIT];
+  const int tid = threadIdx.x;
+  const i32x4 pc = pieces[pc0 + blockIdx.x];  // {block, first chunk, end chunk, piece}
+  const int blk = pc.x, ca = pc.y, ce = pc.z, piece = pc.w;
+  const i32x4 d = F.desc[blk];
+  const int r0 = d.x, nr = d.y;
+  const i32x4* csrc = (const i32x4*)F.xci + d.z;
+  const V4* vsrc = (const V4*)F.xv + d.z;
+  i32x4 cv[R];
+  V4 vv[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int c = ca + u * NT + tid;
+    const int cc = c < ce ? c : ca;
+    cv[u] = __builtin_nontemporal_load(csrc + cc);
+    vv[u] = __builtin_nontemporal_load(vsrc + cc);
+  }
+  int cs = 0, cn = 0;
+  if (tid < nr) {
+    const uint32_t rg = (uint32_t)F.rng[r0 + tid];
+    cs = (int)(rg & 0xffffu);
+    cn = (int)(rg >> 16);
+  }
+  TY yv[R][4];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    yv[u][0] = yin[cv[u].x];
+    yv[u][1] = yin[cv[u].y];
+    yv[u][2] = yin[cv[u].z];
+    yv[u][3] = yin[cv[u].w];
+  }
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int c = ca + u * NT + tid;
+    if (c < ce) {
+      T s = (T)vv[u].x * (T)yv[u][0];
+      s += (T)vv[u].y * (T)yv[u][1];
+      s += (T)vv[u].z * (T)yv[u][2];
+      s += (T)vv[u].w * (T)yv[u][3];
+      part[c - ca] = s;
+    }
+  }
+  __syncthreads();
+  if (tid < nr) {
+    const int2 sp = F.split[blk];
+    const int lo = cs > ca ? cs : ca, hi = cs + cn < ce ? cs + cn : ce;
+    T s0 = 0, s1 = 0;
+    for (int k = lo; k < hi; ++k) {
+      if ((k - cs) & 1) s1 += part[k - ca];
+      else s0 += part[k - ca];
+    }
+    ((T*)F.part)[sp.y + (size_t)piece * nr + tid] = s0 + s1;
+  }
 }
 
 // Lane gl's share of row i of a packed inverse times t, columns gl, gl + G,
@@ -761,7 +834,9 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   const int tid = threadIdx.x;
   const int blk = blk0 + blockIdx.x;
   const i32x4 d = F.desc[blk];
-  const int r0 = d.x, nr = d.y, c0 = d.z, nc = d.w;
+  const int2 sp = F.split[blk];
+  // a split block's external sums come from the partials
+  const int r0 = d.x, nr = d.y, c0 = d.z, nc = sp.x > 1 ? 0 : d.w;
   const int64_t mo = F.inv_off[blk];
   const int nv = (nr * (nr + 1) / 2 + VN - 1) / VN;
   const i32x4* isrc = (const i32x4*)(F.inv + mo);
@@ -797,6 +872,10 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
     bi = (T)b[r0 + tid];
+    if (sp.x > 1) {  // partials in piece order
+      const T* P = (const T*)F.part + sp.y + tid;
+      for (int q = 0; q < sp.x; ++q) bi -= P[(size_t)q * nr];
+    }
   }
 #pragma unroll
   for (int u = 0; u < RI; ++u) {
@@ -845,23 +924,35 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   T s0 = 0, s1 = 0;
   for (int cb = 0; cb < nc; cb += part_cap) {
     const int ce = nc < cb + part_cap ? nc : cb + part_cap;
-    // chunks [cb, ce) not yet summed into part: two per thread in flight
-    for (int c = (cb == 0 ? RC * NT : cb) + tid; c < ce; c += 2 * NT) {
-      const int c1 = c + NT < ce ? c + NT : c;
-      const i32x4 ca = __builtin_nontemporal_load(csrc + c), cc = __builtin_nontemporal_load(csrc + c1);
-      const V4 va = __builtin_nontemporal_load(vsrc + c), vb = __builtin_nontemporal_load(vsrc + c1);
-      const TY ya0 = yin[ca.x], ya1 = yin[ca.y], ya2 = yin[ca.z], ya3 = yin[ca.w];
-      const TY yb0 = yin[cc.x], yb1 = yin[cc.y], yb2 = yin[cc.z], yb3 = yin[cc.w];
-      T s = (T)va.x * (T)ya0;
-      s += (T)va.y * (T)ya1;
-      s += (T)va.z * (T)ya2;
-      s += (T)va.w * (T)ya3;
-      part[c - cb] = s;
-      s = (T)vb.x * (T)yb0;
-      s += (T)vb.y * (T)yb1;
-      s += (T)vb.z * (T)yb2;
-      s += (T)vb.w * (T)yb3;
-      part[c1 - cb] = s;
+    // chunks [cb, ce) not yet summed into part: RR per thread in flight
+    // (out-of-range ones repeat the thread's first chunk of the round)
+    constexpr int RR = 4;
+    for (int c = (cb == 0 ? RC * NT : cb) + tid; c < ce; c += RR * NT) {
+      i32x4 cc[RR];
+      V4 vr[RR];
+      TY yr[RR][4];
+#pragma unroll
+      for (int u = 0; u < RR; ++u) {
+        const int cu = c + u * NT < ce ? c + u * NT : c;
+        cc[u] = __builtin_nontemporal_load(csrc + cu);
+        vr[u] = __builtin_nontemporal_load(vsrc + cu);
+      }
+#pragma unroll
+      for (int u = 0; u < RR; ++u) {
+        yr[u][0] = yin[cc[u].x];
+        yr[u][1] = yin[cc[u].y];
+        yr[u][2] = yin[cc[u].z];
+        yr[u][3] = yin[cc[u].w];
+      }
+#pragma unroll
+      for (int u = 0; u < RR; ++u) {
+        const int cu = c + u * NT < ce ? c + u * NT : c;
+        T s = (T)vr[u].x * (T)yr[u][0];
+        s += (T)vr[u].y * (T)yr[u][1];
+        s += (T)vr[u].z * (T)yr[u][2];
+        s += (T)vr[u].w * (T)yr[u][3];
+        part[cu - cb] = s;
+      }
     }
     __syncthreads();
     const int lo = cs > cb ? cs : cb, hi = cs + cn < ce ? cs + cn : ce;
@@ -953,8 +1044,10 @@ __host__ __device__ inline int ilu_numeric_hash_size(int max_len) {
                                    // fill entries stay short
   return h;
 }
+// working row (plus one slot that absorbs the updates of columns the row
+// does not store), per-L-column U spans and pivots, columns, hash map
 __host__ __device__ inline size_t ilu_numeric_lds_bytes(int max_len) {
-  return (size_t)max_len * (8 + 8 + 8 + 4 + 4) + (size_t)ilu_numeric_hash_size(max_len) * 8;
+  return (size_t)(max_len + 2) * 8 + (size_t)max_len * (8 + 8 + 4 + 4) + (size_t)ilu_numeric_hash_size(max_len) * 8;
 }
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -965,12 +1058,17 @@ __device__ __forceinline__ void lds_barrier() {
 __device__ __forceinline__ uint32_t ilu_hash(int32_t c, uint32_t mask) {
   return ((uint32_t)c * 2654435769u) >> (32 - __builtin_popcount(mask));
 }
-__device__ __forceinline__ int ilu_find(const int32_t* keys, const int32_t* vals, uint32_t mask, int32_t c) {
-  for (uint32_t h = ilu_hash(c, mask);; h = (h + 1) & mask) {
-    const int32_t k = keys[h];
-    if (k == c) return vals[h];
-    if (k < 0) return -1;
+// hash map slots {column, position}, linear probing; column -1 = empty
+__device__ __forceinline__ int ilu_probe(const int2* tab, uint32_t mask, int32_t c, uint32_t h, int2 e) {
+  while (e.x != c && e.x >= 0) {
+    h = (h + 1) & mask;
+    e = tab[h];
   }
+  return e.x == c ? e.y : -1;
+}
+__device__ __forceinline__ int ilu_find(const int2* tab, uint32_t mask, int32_t c) {
+  const uint32_t h = ilu_hash(c, mask);
+  return ilu_probe(tab, mask, c, h, tab[h]);
 }
 template <int NT, int M>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int blk0, int blk1,
@@ -981,13 +1079,12 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const int hs = ilu_numeric_hash_size(max_len);
   const uint32_t mask = (uint32_t)hs - 1;
-  double* w = (double*)smem;                 // working row
-  int64_t* qs = (int64_t*)(w + max_len);     // per L column k: first U entry of row k
+  double* w = (double*)smem;                 // working row; w[max_len]: dump slot
+  int64_t* qs = (int64_t*)(w + max_len + 2); // per L column k: first U entry of row k
   double* uk = (double*)(qs + max_len);      // per L column k: pivot u_kk
-  int32_t* lc = (int32_t*)(uk + max_len);    // columns of the row
+  int2* tab = (int2*)(uk + max_len);         // hash map column -> position
+  int32_t* lc = (int32_t*)(tab + hs);        // columns of the row
   int32_t* qn = lc + max_len;                // per L column k: U entries of row k
-  int32_t* keys = qn + max_len;              // hash map column -> position
-  int32_t* vals = keys + hs;
   const int tid = threadIdx.x;
   const int blk = blk0 + blockIdx.x;
   if (blk >= blk1) return;
@@ -996,15 +1093,15 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     const int64_t base = F.rp[r];
     const int len = (int)(F.rp[r + 1] - base);
     const int nl = F.o_diag[r];
-    for (int h = tid; h < hs; h += NT) keys[h] = -1;
+    for (int h = tid; h < hs; h += NT) tab[h] = make_int2(-1, 0);
     lds_barrier();
     for (int t = tid; t < len; t += NT) {
       const int32_t c = F.ci[base + t];
       lc[t] = c;
       w[t] = 0.0;
       for (uint32_t h = ilu_hash(c, mask);; h = (h + 1) & mask)
-        if (atomicCAS(&keys[h], -1, c) == -1) {
-          vals[h] = t;
+        if (atomicCAS(&tab[h].x, -1, c) == -1) {
+          tab[h].y = t;
           break;
         }
       if (t < nl) {
@@ -1019,7 +1116,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     for (int64_t p = srp[r] + tid; p < srp[r + 1]; p += NT) {
       const int32_t c = sci[p];
       if (c >= col_limit) continue;
-      const int pos = ilu_find(keys, vals, mask, c);
+      const int pos = ilu_find(tab, mask, c);
       if (pos >= 0) w[pos] = sv[p];
     }
     lds_barrier();
@@ -1048,15 +1145,30 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
       // w[t] is final: later L columns only update positions > t
       const double mult = w[t] / uk[t];
       const int n = qn[t];
+      // first probes of all M lookups together, then the rare collisions;
+      // entries out of range or not stored update the dump slot, so the
+      // reads and writes of w issue back to back
+      uint32_t h[M];
+      int2 e[M];
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        h[m] = ilu_hash(j[m], mask);
+        e[m] = tab[h[m]];
+      }
       int pos[M];
 #pragma unroll
-      for (int m = 0; m < M; ++m) pos[m] = tid + m * NT < n ? ilu_find(keys, vals, mask, j[m]) : -1;
+      for (int m = 0; m < M; ++m) {
+        const int p = ilu_probe(tab, mask, j[m], h[m], e[m]);
+        pos[m] = tid + m * NT < n && p >= 0 ? p : max_len;
+      }
+      double o[M];
 #pragma unroll
-      for (int m = 0; m < M; ++m)
-        if (pos[m] >= 0) w[pos[m]] -= mult * v[m];
+      for (int m = 0; m < M; ++m) o[m] = w[pos[m]];
+#pragma unroll
+      for (int m = 0; m < M; ++m) w[pos[m]] = o[m] - mult * v[m];
       const int64_t q = qs[t];
       for (int idx = M * NT + tid; idx < n; idx += NT) {  // U parts longer than M NT
-        const int p = ilu_find(keys, vals, mask, F.ci[q + idx]);
+        const int p = ilu_find(tab, mask, F.ci[q + idx]);
         if (p >= 0) w[p] -= mult * F.v[q + idx];
       }
     };

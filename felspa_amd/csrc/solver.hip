@@ -307,9 +307,10 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     // chunks}; per row its first chunk relative to the block (low 16 bits)
     // and its chunk count (high 16 bits).
     auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& dci, DevBuf& dpos, DevBuf& dv64, DevBuf& dv32,
-                   int64_t& n_ent, int& c_max) {
+                   int64_t& n_ent, int& c_max, std::vector<int32_t>& dsc) {
       const int nb = sched.n_blocks();
-      std::vector<int32_t> dsc(4 * (size_t)nb), rg(n);
+      std::vector<int32_t> rg(n);
+      dsc.assign(4 * (size_t)nb, 0);
       int64_t q = 0;  // chunks so far
       c_max = 0;
       for (int b = 0; b < nb; ++b) {
@@ -352,8 +353,44 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       dv32.alloc(4 * cc.size());
     };
     n_ext = sum_lext + sum_uext;
-    ext(true, xldesc, xlrng, xlci, xlpos, xlv64, xlv32, n_xl, xl_max);
-    ext(false, xudesc, xurng, xuci, xupos, xuv64, xuv32, n_xu, xu_max);
+    std::vector<int32_t> dl, du;
+    ext(true, xldesc, xlrng, xlci, xlpos, xlv64, xlv32, n_xl, xl_max, dl);
+    ext(false, xudesc, xurng, xuci, xupos, xuv64, xuv32, n_xu, xu_max, du);
+    // blocks with many external chunks are summed in pieces of DENSE_SPLIT
+    // chunks by workgroups of their own; pieces listed stage by stage
+    int64_t part_need = 0;
+    auto split = [&](const std::vector<int32_t>& dsc, DevBuf& dsplit, DevBuf& dpieces, std::vector<int>& pc_ptr) {
+      const int nb = sched.n_blocks();
+      std::vector<int32_t> sp(2 * (size_t)nb), pcs;
+      pc_ptr.assign(1, 0);
+      int64_t off = 0;
+      for (int st = 0; st < sched.n_stages(); ++st) {
+        for (int b = sched.stage_ptr[st]; b < sched.stage_ptr[st + 1]; ++b) {
+          const int nr = dsc[4 * b + 1], nc = dsc[4 * b + 3];
+          const int S = nc > 2 * DENSE_SPLIT ? (nc + DENSE_SPLIT - 1) / DENSE_SPLIT : 1;
+          sp[2 * b] = S;
+          sp[2 * b + 1] = (int32_t)off;
+          if (S > 1) {
+            for (int q = 0; q < S; ++q) {
+              pcs.push_back(b);
+              pcs.push_back(q * DENSE_SPLIT);
+              pcs.push_back(std::min(nc, (q + 1) * DENSE_SPLIT));
+              pcs.push_back(q);
+            }
+            off += (int64_t)S * nr;
+            if (off > INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external entries");
+          }
+        }
+        pc_ptr.push_back((int)(pcs.size() / 4));
+      }
+      if (pcs.empty()) pcs.assign(4, 0);
+      dsplit.upload(sp.data(), 4 * sp.size());
+      dpieces.upload(pcs.data(), 4 * pcs.size());
+      part_need = std::max(part_need, off);
+    };
+    split(dl, xlsplit, xlpieces, xl_pc_ptr);
+    split(du, xusplit, xupieces, xu_pc_ptr);
+    dpart.alloc(8 * std::max<int64_t>(1, part_need));
     // the last stage has no external U entries when no two blocks of a stage
     // couple: its backward solve is then in-block only and runs in the same
     // launch as its forward solve
@@ -504,6 +541,8 @@ static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY*
   D.xv = fwd ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
   D.inv_off = F.inv_off.as<int64_t>();
   D.inv = fwd ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
+  D.split = fwd ? F.xlsplit.as<int2>() : F.xusplit.as<int2>();
+  D.part = F.dpart.p;
   const TV* uinv = MODE == DS_BOTH ? (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p) : nullptr;
   // partial sums of every chunk of the largest block when the workgroup's
   // LDS stays within 40 KiB (several workgroups per CU), else chunks in
@@ -526,14 +565,34 @@ static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY*
 
 template <typename TV, typename T, typename TB, typename TY, int MODE>
 static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+  // pieces of the stage's split blocks first
+  const bool fwd = MODE != DS_BWD;
+  const std::vector<int>& pp = fwd ? F.xl_pc_ptr : F.xu_pc_ptr;
+  int s = 0;
+  while (F.sched.stage_ptr[s] != b0) ++s;
+  if (pp[s + 1] > pp[s]) {
+    const bool d = std::is_same<TV, double>::value;
+    DenseTri<TV> D;
+    D.desc = fwd ? F.xldesc.as<i32x4>() : F.xudesc.as<i32x4>();
+    D.rng = fwd ? F.xlrng.as<int32_t>() : F.xurng.as<int32_t>();
+    D.xci = fwd ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
+    D.xv = fwd ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
+    D.inv_off = nullptr;
+    D.inv = nullptr;
+    D.split = fwd ? F.xlsplit.as<int2>() : F.xusplit.as<int2>();
+    D.part = F.dpart.p;
+    hipLaunchKernelGGL((k_ilu_ext_partials<TV, T, TY>), dim3(pp[s + 1] - pp[s]), dim3(DENSE_SPLIT_NT), 0, st, D,
+                       fwd ? F.xlpieces.as<i32x4>() : F.xupieces.as<i32x4>(), pp[s], yin);
+  }
   // 128 threads for blocks of up to 64 rows (more workgroups resident per
-  // CU), 256 above; G threads per row in the in-block product
+  // CU), 512 above (larger blocks carry many more external entries: more of
+  // them in flight per block); G threads per row in the in-block product
   const int mr = F.sched.max_block_rows;
   if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE>(F, b0, b1, b, yin, y, st);
   else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE>(F, b0, b1, b, yin, y, st);
   else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 128) launch_dense<TV, T, TB, TY, 256, 2, MODE>(F, b0, b1, b, yin, y, st);
-  else launch_dense<TV, T, TB, TY, 256, 1, MODE>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 128) launch_dense<TV, T, TB, TY, 512, 4, MODE>(F, b0, b1, b, yin, y, st);
+  else launch_dense<TV, T, TB, TY, 512, 2, MODE>(F, b0, b1, b, yin, y, st);
 }
 
 template <typename TV, typename T, typename TB>
@@ -557,9 +616,9 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) con
   const int ns = sched.n_stages();
   // the dense kernels keep a block's packed triangle (the fused last stage
   // both of them) and its chunk partial sums in LDS
-  // (room for the first round's partial sums of 256 threads)
+  // (room for the first round's partial sums of 512 threads)
   const size_t lim = dense_wg_lds_limit();
-  const int pc = 256 * dense_rc<TV, T>();
+  const int pc = 512 * dense_rc<TV, T>();
   if (dense && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
     // (FP32 factors only: with FP64 ones both triangles' registers cost more
     // occupancy than the launch saves)

@@ -82,6 +82,8 @@ struct IluDev {
   bool fuse_last = false;          // last stage: forward and backward in one launch
   DevBuf inv_off, linv64, uinv64, linv32, uinv32;
   DevBuf xldesc, xlrng, xlci, xlpos, xlv64, xlv32, xudesc, xurng, xuci, xupos, xuv64, xuv32;
+  DevBuf xlsplit, xusplit, xlpieces, xupieces, dpart;  // pieces of blocks with many external chunks
+  std::vector<int> xl_pc_ptr, xu_pc_ptr;               // per stage: range of pieces
   void gather_inblock(hipStream_t st);
   void build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max);
   template <typename T>

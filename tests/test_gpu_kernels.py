@@ -74,6 +74,43 @@ def test_ilu_block_color(oracle, cfg, n, k):
     assert rel(ilu.Tvmult(b), wantT) < 1e-12
 
 
+@pytest.mark.parametrize("cfg,n,hint", [("C3", 6, 3), ("C2", 24, 5)])
+def test_ilu_dense_stages_large_blocks(oracle, cfg, n, hint):
+    """ILU(1) in lattice blocks as the solver uses them (3D: 81-row blocks
+    whose thousands of external chunks are summed in pieces by several
+    workgroups); FP64 and FP32 applications against the oracle."""
+    S = system(cfg, n)
+    A = S.blocks["A"]
+    ilu = fa.SparseILU(ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=S.block_hint(hint))
+    ilu.initialize(A, fa.SparseILU.AdditionalData(level_of_fill=1))
+    F, perm = ilu.factors()
+    PA = _perm_matrix(A, perm)
+    ref = oracle.SparseILU(PA, 1)
+    R = ref.factors()
+    assert (F.indptr == R.indptr).all() and (F.indices == R.indices).all()
+    assert np.abs(F.data - R.data).max() <= 1e-12 * np.abs(R.data).max()
+    # largest block's count of 4-entry chunks of external L entries: above
+    # 1536 its sums are split into pieces
+    h = np.asarray(S.block_hint(hint))[perm]
+    starts = np.flatnonzero(np.r_[True, h[1:] != h[:-1]])
+    chunks = []
+    for a, e in zip(starts, np.r_[starts[1:], len(h)]):
+        c = 0
+        for r in range(a, e):
+            cols = F.indices[F.indptr[r]:F.indptr[r + 1]]
+            c += (int((cols < a).sum()) + 3) // 4
+        chunks.append(c)
+    if cfg == "C3":
+        assert max(chunks) > 1536
+    rng = np.random.default_rng(4)
+    for trial in range(2):  # the second application reuses the pieces' buffers
+        b = rng.standard_normal(A.shape[0])
+        want = np.empty_like(b)
+        want[perm] = ref.vmult(b[perm])
+        assert rel(ilu.vmult(b), want) < 1e-12
+        assert rel(ilu.vmult(b, precision=L.FS_PREC_MIXED), want) < 1e-4
+
+
 def test_ilu_zero_pivot_reported():
     import scipy.sparse as sp
     A = sp.csr_matrix(np.array([[0.0, 1.0], [1.0, 2.0]]))
