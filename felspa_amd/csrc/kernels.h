@@ -672,13 +672,31 @@ template <typename T>
 struct DenseTri {
   const i32x4* desc;      // per block: first row, rows, first chunk, chunks
   const int32_t* rng;     // per row: first chunk in the block | chunks << 16
-  const int32_t* xci;     // external columns, chunks of 4
+  const void* xidx;       // per external entry: its column's index in the block's halo (IB bytes)
   const T* xv;            // external values, chunks of 4
+  const int32_t* hcol;    // halo columns, every block's from a multiple of 4
+  const int2* halo;       // per block: {first halo column, halo columns}
   const int64_t* inv_off; // per block: start of its packed triangle(s)
   const T* inv;           // packed inverse of the in-block factor
   const int2* split;      // per block: {S, first partial}; S > 1: its chunks
                           // were summed by S workgroups of k_ilu_ext_partials
   void* part;             // row partial sums of split blocks, S x rows each
+};
+
+// 4 halo indices of one chunk: IB = 1 byte each (one word) or 2 (two words)
+template <int IB>
+struct Idx4;
+template <>
+struct Idx4<1> {
+  typedef uint32_t type;
+  static __device__ __forceinline__ int get(type v, int e) { return (int)((v >> (8 * e)) & 0xffu); }
+};
+template <>
+struct Idx4<2> {
+  typedef uint32_t type __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ int get(type v, int e) {
+    return (int)(((e < 2 ? v.x : v.y) >> (16 * (e & 1))) & 0xffffu);
+  }
 };
 
 // Blocks with more than 2 DENSE_SPLIT chunks are summed in pieces of
@@ -693,31 +711,76 @@ __host__ __device__ constexpr int dense_rc() {
   return sizeof(TV) + sizeof(TY) <= 8 ? 5 : 3;
 }
 
+// The block's halo values (T) into LDS hv: thread k gathers halo columns
+// 4k..4k+3, 4(k + NT).., U groups of 4 in flight, from the finished values yin
+template <typename T, typename TY, int NT, int U>
+__device__ __forceinline__ void gather_halo(const int32_t* __restrict__ hcol, int nh, const TY* yin, T* hv,
+                                            int tid) {
+  const i32x4* h4 = (const i32x4*)hcol;
+  const int nu = (nh + 3) / 4;
+  for (int k = tid; k < nu; k += U * NT) {
+    i32x4 c[U];
+    TY v[U][4];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(h4 + (k + u * NT < nu ? k + u * NT : k));
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      v[u][0] = yin[c[u].x];
+      v[u][1] = yin[c[u].y];
+      v[u][2] = yin[c[u].z];
+      v[u][3] = yin[c[u].w];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int ku = k + u * NT < nu ? k + u * NT : k;
+      hv[4 * ku] = (T)v[u][0];
+      hv[4 * ku + 1] = (T)v[u][1];
+      hv[4 * ku + 2] = (T)v[u][2];
+      hv[4 * ku + 3] = (T)v[u][3];
+    }
+  }
+}
+
+// one chunk's sum v . hv[idx] in entry order
+template <typename T, typename V4, int IB>
+__device__ __forceinline__ T chunk_dot(const V4& v, typename Idx4<IB>::type ix, const T* hv) {
+  T s = (T)v.x * hv[Idx4<IB>::get(ix, 0)];
+  s += (T)v.y * hv[Idx4<IB>::get(ix, 1)];
+  s += (T)v.z * hv[Idx4<IB>::get(ix, 2)];
+  s += (T)v.w * hv[Idx4<IB>::get(ix, 3)];
+  return s;
+}
+
 // External sums of one piece (chunks [ca, ce) of block blk, ce - ca <=
-// DENSE_SPLIT) of a block with many external entries: every row's sum over
-// its chunks inside the piece, in chunk order, to part[first + piece rows +
-// row]. One workgroup per piece, all loads in one round.
-template <typename TV, typename T, typename TY>
+// DENSE_SPLIT) of a block with many external entries: the block's halo is
+// gathered into LDS, then every row's sum over its chunks inside the piece,
+// in chunk order, goes to part[first + piece rows + row]. One workgroup per
+// piece, all chunk loads in one round.
+template <typename TV, typename T, typename TY, int IB>
 __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV> F, const i32x4* __restrict__ pieces,
                                                                      int pc0, const TY* yin) {
   typedef TV V4 __attribute__((ext_vector_type(4)));
+  typedef typename Idx4<IB>::type IX;
   constexpr int NT = DENSE_SPLIT_NT;
   constexpr int R = (DENSE_SPLIT + NT - 1) / NT;
-  __shared__ T part[DENSE_SPLIT];
+  extern __shared__ __attribute__((aligned(16))) char dsm[];
+  T* part = (T*)dsm;
+  T* hv = part + DENSE_SPLIT;
   const int tid = threadIdx.x;
   const i32x4 pc = pieces[pc0 + blockIdx.x];  // {block, first chunk, end chunk, piece}
   const int blk = pc.x, ca = pc.y, ce = pc.z, piece = pc.w;
   const i32x4 d = F.desc[blk];
+  const int2 hl = F.halo[blk];
   const int r0 = d.x, nr = d.y;
-  const i32x4* csrc = (const i32x4*)F.xci + d.z;
+  const IX* isrc = (const IX*)F.xidx + d.z;
   const V4* vsrc = (const V4*)F.xv + d.z;
-  i32x4 cv[R];
+  IX ix[R];
   V4 vv[R];
 #pragma unroll
   for (int u = 0; u < R; ++u) {
     const int c = ca + u * NT + tid;
     const int cc = c < ce ? c : ca;
-    cv[u] = __builtin_nontemporal_load(csrc + cc);
+    ix[u] = __builtin_nontemporal_load(isrc + cc);
     vv[u] = __builtin_nontemporal_load(vsrc + cc);
   }
   int cs = 0, cn = 0;
@@ -726,24 +789,12 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
   }
-  TY yv[R][4];
-#pragma unroll
-  for (int u = 0; u < R; ++u) {
-    yv[u][0] = yin[cv[u].x];
-    yv[u][1] = yin[cv[u].y];
-    yv[u][2] = yin[cv[u].z];
-    yv[u][3] = yin[cv[u].w];
-  }
+  gather_halo<T, TY, NT, 2>(F.hcol + hl.x, hl.y, yin, hv, tid);
+  __syncthreads();
 #pragma unroll
   for (int u = 0; u < R; ++u) {
     const int c = ca + u * NT + tid;
-    if (c < ce) {
-      T s = (T)vv[u].x * (T)yv[u][0];
-      s += (T)vv[u].y * (T)yv[u][1];
-      s += (T)vv[u].z * (T)yv[u][2];
-      s += (T)vv[u].w * (T)yv[u][3];
-      part[c - ca] = s;
-    }
+    if (c < ce) part[c - ca] = chunk_dot<T, V4, IB>(vv[u], ix[u], hv);
   }
   __syncthreads();
   if (tid < nr) {
@@ -796,31 +847,36 @@ __device__ __forceinline__ T upper_row_dot(const TV* M, const T* t, int n, int i
 // entries: every later column of a last-stage row lies in its own block).
 enum { DS_FWD = 0, DS_BWD = 1, DS_BOTH = 2 };
 
-// LDS of one dense stage workgroup: packed inverse(s) (tv bytes), one
-// partial sum per external chunk and the right-hand side(s) (ta bytes)
-__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int part_cap, int tv, int ta, int mode) {
+// LDS of one dense stage workgroup: packed inverse(s) (tv bytes), the
+// block's halo values, one partial sum per external chunk and the
+// right-hand side(s) (ta bytes)
+__host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int hcap, int part_cap, int tv, int ta, int mode) {
   const int ninv = mode == DS_BOTH ? 2 : 1;
-  return (size_t)tv * ninv * inv_cap + (size_t)ta * (part_cap + 256 * ninv);
+  return (size_t)tv * ninv * inv_cap + (size_t)ta * (hcap + part_cap + 256 * ninv);
 }
 
 // One stage, one block per workgroup of NT threads. The block's external
-// entries are chunks of 4 (every row's list padded with value-0 entries), so
-// thread k reads chunks k, k + NT, ... as 16-byte column and value vectors,
-// consecutive threads at consecutive addresses, with no per-row pointer in
-// the way; the packed inverse(s) (starting at multiples of 4 elements) load
-// as 16-byte units at the same time:
-//  1. issue the first RI inverse units and RC chunks of every thread, then
-//     the gathers yin[col] of its chunks; chunk partial sums go to LDS;
-//  2. per row: t_i = b_i - (sum of the row's chunk partials in order);
+// entries are chunks of 4 (every row's list padded with value-0 entries),
+// each entry a value and the index of its column in the block's halo (the
+// block's distinct external columns); thread k reads chunks k, k + NT, ... as
+// 16-byte value vectors and 4- or 8-byte index words, consecutive threads at
+// consecutive addresses; the packed inverse(s) (starting at multiples of 4
+// elements) and the halo columns load in the same round:
+//  1. issue the first RI inverse units, RC chunks and the halo columns of
+//     every thread; then gather the halo's finished values yin[col] into LDS;
+//  2. chunk sums v . halo[idx] into LDS; per row: t_i = b_i - (sum of the
+//     row's chunk sums in order), in passes of part_cap chunks;
 //  3. G threads per row: y_i = sum_j inv(i, j) t_j from LDS (MODE 2 then
 //     y_i = sum_j U_inv(i, j) t'_j).
 // The order of every sum is fixed, so results do not depend on scheduling.
-template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE>
+// Blocks split into pieces (S > 1) take their external sums from the pieces.
+template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE, int IB>
 __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV* __restrict__ uinv, int blk0,
-                                                        int inv_cap, int part_cap, const TB* b, const TY* yin,
-                                                        TY* y) {
+                                                        int inv_cap, int hcap, int part_cap, const TB* b,
+                                                        const TY* yin, TY* y) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
   typedef TV V4 __attribute__((ext_vector_type(4)));
+  typedef typename Idx4<IB>::type IX;
   constexpr int VN = 16 / sizeof(TV);
   constexpr bool FWD = MODE != DS_BWD;
   constexpr int NINV = MODE == DS_BOTH ? 2 : 1;
@@ -828,21 +884,24 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   constexpr int RC = dense_rc<TV, TY>();         // chunks per thread, first round
   TV* Ms = (TV*)dsm;
   TV* Us = Ms + inv_cap;
-  T* part = (T*)(Ms + NINV * inv_cap);
+  T* hv = (T*)(Ms + NINV * inv_cap);
+  T* part = hv + hcap;
   T* t = part + part_cap;
   T* t2 = t + 256;
   const int tid = threadIdx.x;
   const int blk = blk0 + blockIdx.x;
   const i32x4 d = F.desc[blk];
   const int2 sp = F.split[blk];
-  // a split block's external sums come from the partials
-  const int r0 = d.x, nr = d.y, c0 = d.z, nc = sp.x > 1 ? 0 : d.w;
+  const int2 hl = F.halo[blk];
+  // a split block's external sums come from the pieces
+  const bool whole = sp.x <= 1;
+  const int r0 = d.x, nr = d.y, nc = whole ? d.w : 0, nh = whole ? hl.y : 0;
   const int64_t mo = F.inv_off[blk];
   const int nv = (nr * (nr + 1) / 2 + VN - 1) / VN;
   const i32x4* isrc = (const i32x4*)(F.inv + mo);
   const i32x4* usrc = MODE == DS_BOTH ? (const i32x4*)(uinv + mo) : nullptr;
-  const i32x4* csrc = (const i32x4*)F.xci + c0;
-  const V4* vsrc = (const V4*)F.xv + c0;
+  const IX* xsrc = (const IX*)F.xidx + d.z;
+  const V4* vsrc = (const V4*)F.xv + d.z;
   i32x4 iv[RI], uv[RI];
 #pragma unroll
   for (int u = 0; u < RI; ++u) {
@@ -852,15 +911,13 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (MODE == DS_BOTH) uv[u] = __builtin_nontemporal_load(usrc + k);
     }
   }
-  // chunks in passes of part_cap (>= RC NT) partial sums; the first RC NT
-  // chunks load with the inverse
-  i32x4 cv[RC];
+  IX cx[RC];
   V4 vv[RC];
 #pragma unroll
   for (int u = 0; u < RC; ++u) {
     const int c = u * NT + tid;
     if (c < nc) {
-      cv[u] = __builtin_nontemporal_load(csrc + c);
+      cx[u] = __builtin_nontemporal_load(xsrc + c);
       vv[u] = __builtin_nontemporal_load(vsrc + c);
     }
   }
@@ -872,7 +929,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
     bi = (T)b[r0 + tid];
-    if (sp.x > 1) {  // partials in piece order
+    if (!whole) {  // partials in piece order
       const T* P = (const T*)F.part + sp.y + tid;
       for (int q = 0; q < sp.x; ++q) bi -= P[(size_t)q * nr];
     }
@@ -885,26 +942,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (MODE == DS_BOTH) ((i32x4*)Us)[k] = uv[u];
     }
   }
-  TY yv[RC][4];
-#pragma unroll
-  for (int u = 0; u < RC; ++u)
-    if (u * NT + tid < nc) {
-      yv[u][0] = yin[cv[u].x];
-      yv[u][1] = yin[cv[u].y];
-      yv[u][2] = yin[cv[u].z];
-      yv[u][3] = yin[cv[u].w];
-    }
-#pragma unroll
-  for (int u = 0; u < RC; ++u) {
-    const int c = u * NT + tid;
-    if (c < nc) {
-      T s = (T)vv[u].x * (T)yv[u][0];
-      s += (T)vv[u].y * (T)yv[u][1];
-      s += (T)vv[u].z * (T)yv[u][2];
-      s += (T)vv[u].w * (T)yv[u][3];
-      part[c] = s;
-    }
-  }
+  gather_halo<T, TY, NT, (NT >= 512 ? 2 : 1)>(F.hcol + hl.x, nh, yin, hv, tid);
   // the rest of the inverse: two units per thread in flight
   for (int k = RI * NT + tid; k < nv; k += 2 * NT) {
     const bool two = k + NT < nv;
@@ -919,39 +957,36 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (two) ((i32x4*)Us)[k + NT] = u1;
     }
   }
-  // row sums in chunk order, two partial sums by the parity of the chunk
-  // within its row (the same split whatever the passes)
+  __syncthreads();
+  // chunk sums in passes of part_cap (>= RC NT) partial sums; row sums in
+  // chunk order, two partial sums by the parity of the chunk within its row
   T s0 = 0, s1 = 0;
   for (int cb = 0; cb < nc; cb += part_cap) {
     const int ce = nc < cb + part_cap ? nc : cb + part_cap;
+    if (cb == 0) {
+#pragma unroll
+      for (int u = 0; u < RC; ++u) {
+        const int c = u * NT + tid;
+        if (c < nc) part[c] = chunk_dot<T, V4, IB>(vv[u], cx[u], hv);
+      }
+    }
     // chunks [cb, ce) not yet summed into part: RR per thread in flight
-    // (out-of-range ones repeat the thread's first chunk of the round)
-    constexpr int RR = 4;
+    // (2 for FP32, which keeps the 128-thread workgroups of a 2D stage
+    // resident in one round)
+    constexpr int RR = sizeof(TV) == 4 && sizeof(TY) == 4 ? 2 : 4;
     for (int c = (cb == 0 ? RC * NT : cb) + tid; c < ce; c += RR * NT) {
-      i32x4 cc[RR];
+      IX xr[RR];
       V4 vr[RR];
-      TY yr[RR][4];
 #pragma unroll
       for (int u = 0; u < RR; ++u) {
         const int cu = c + u * NT < ce ? c + u * NT : c;
-        cc[u] = __builtin_nontemporal_load(csrc + cu);
+        xr[u] = __builtin_nontemporal_load(xsrc + cu);
         vr[u] = __builtin_nontemporal_load(vsrc + cu);
       }
 #pragma unroll
       for (int u = 0; u < RR; ++u) {
-        yr[u][0] = yin[cc[u].x];
-        yr[u][1] = yin[cc[u].y];
-        yr[u][2] = yin[cc[u].z];
-        yr[u][3] = yin[cc[u].w];
-      }
-#pragma unroll
-      for (int u = 0; u < RR; ++u) {
         const int cu = c + u * NT < ce ? c + u * NT : c;
-        T s = (T)vr[u].x * (T)yr[u][0];
-        s += (T)vr[u].y * (T)yr[u][1];
-        s += (T)vr[u].z * (T)yr[u][2];
-        s += (T)vr[u].w * (T)yr[u][3];
-        part[cu - cb] = s;
+        part[cu - cb] = chunk_dot<T, V4, IB>(vr[u], xr[u], hv);
       }
     }
     __syncthreads();

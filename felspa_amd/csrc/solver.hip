@@ -306,8 +306,9 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     // block after each other. Per block {first row, rows, first chunk,
     // chunks}; per row its first chunk relative to the block (low 16 bits)
     // and its chunk count (high 16 bits).
-    auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& dci, DevBuf& dpos, DevBuf& dv64, DevBuf& dv32,
-                   int64_t& n_ent, int& c_max, std::vector<int32_t>& dsc) {
+    auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& didx, DevBuf& dhcol, DevBuf& dhalo, DevBuf& dpos,
+                   DevBuf& dv64, DevBuf& dv32, int64_t& n_ent, int& c_max, std::vector<int32_t>& dsc, int& ib,
+                   int& h_max, int64_t& n_halo) {
       const int nb = sched.n_blocks();
       std::vector<int32_t> rg(n);
       dsc.assign(4 * (size_t)nb, 0);
@@ -345,17 +346,56 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
           }
           for (; (e - e0) % 4; ++e) cc[e] = cc[e0];
         }
+      // the block's halo: its distinct external columns, ascending (nearby
+      // lanes gather nearby values); every entry stores its column's index in
+      // the halo (1 byte when no halo exceeds 256 columns, else 2); halos
+      // start at multiples of 4
+      std::vector<int32_t> hc, hl(2 * (size_t)nb), hb;
+      std::vector<int32_t> loc(cc.size(), 0), stamp(n, -1), slot(n, 0);
+      h_max = 0;
+      for (int b = 0; b < nb; ++b) {
+        const int64_t a = 4 * (int64_t)dsc[4 * b + 2], z = a + 4 * (int64_t)dsc[4 * b + 3];
+        hb.clear();
+        for (int64_t e2 = a; e2 < z; ++e2)
+          if (stamp[cc[e2]] != b) {
+            stamp[cc[e2]] = b;
+            hb.push_back(cc[e2]);
+          }
+        std::sort(hb.begin(), hb.end());
+        for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
+        for (int64_t e2 = a; e2 < z; ++e2) loc[e2] = slot[cc[e2]];
+        if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external columns");
+        hl[2 * b] = (int32_t)hc.size();
+        hl[2 * b + 1] = (int32_t)hb.size();
+        h_max = std::max<int>(h_max, (int)hb.size());
+        hc.insert(hc.end(), hb.begin(), hb.end());
+        while (hc.size() % 4) hc.push_back(hb.empty() ? 0 : hb[0]);
+      }
+      if (hc.size() > (size_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external columns");
+      if (hc.empty()) hc.assign(4, 0);
+      n_halo = (int64_t)hc.size();
+      ib = h_max <= 256 ? 1 : 2;
+      if (ib == 1) {
+        std::vector<uint8_t> ix(loc.size());
+        for (size_t e2 = 0; e2 < loc.size(); ++e2) ix[e2] = (uint8_t)loc[e2];
+        didx.upload(ix.data(), ix.size());
+      } else {
+        std::vector<uint16_t> ix(loc.size());
+        for (size_t e2 = 0; e2 < loc.size(); ++e2) ix[e2] = (uint16_t)loc[e2];
+        didx.upload(ix.data(), 2 * ix.size());
+      }
       desc.upload(dsc.data(), 4 * dsc.size());
       rng.upload(rg.data(), 4 * rg.size());
-      dci.upload(cc.data(), 4 * cc.size());
+      dhcol.upload(hc.data(), 4 * hc.size());
+      dhalo.upload(hl.data(), 4 * hl.size());
       dpos.upload(po.data(), 8 * po.size());
       dv64.alloc(8 * cc.size());
       dv32.alloc(4 * cc.size());
     };
     n_ext = sum_lext + sum_uext;
     std::vector<int32_t> dl, du;
-    ext(true, xldesc, xlrng, xlci, xlpos, xlv64, xlv32, n_xl, xl_max, dl);
-    ext(false, xudesc, xurng, xuci, xupos, xuv64, xuv32, n_xu, xu_max, du);
+    ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, n_xlh);
+    ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, n_xuh);
     // blocks with many external chunks are summed in pieces of DENSE_SPLIT
     // chunks by workgroups of their own; pieces listed stage by stage
     int64_t part_need = 0;
@@ -528,71 +568,94 @@ static size_t dense_wg_lds_limit() {
   return lim;
 }
 
-template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE>
-static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
-  const int nb = b1 - b0;
-  if (nb <= 0) return;
+// the dense kernels' view of one triangle (forward: L, backward: U)
+template <typename TV>
+static DenseTri<TV> dense_view(const IluDev& F, bool fwd) {
   const bool d = std::is_same<TV, double>::value;
-  const bool fwd = MODE != DS_BWD;
   DenseTri<TV> D;
   D.desc = fwd ? F.xldesc.as<i32x4>() : F.xudesc.as<i32x4>();
   D.rng = fwd ? F.xlrng.as<int32_t>() : F.xurng.as<int32_t>();
-  D.xci = fwd ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
+  D.xidx = fwd ? F.xlidx.p : F.xuidx.p;
   D.xv = fwd ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
+  D.hcol = fwd ? F.xlhcol.as<int32_t>() : F.xuhcol.as<int32_t>();
+  D.halo = fwd ? F.xlhalo.as<int2>() : F.xuhalo.as<int2>();
   D.inv_off = F.inv_off.as<int64_t>();
   D.inv = fwd ? (d ? (const TV*)F.linv64.p : (const TV*)F.linv32.p) : (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p);
   D.split = fwd ? F.xlsplit.as<int2>() : F.xusplit.as<int2>();
   D.part = F.dpart.p;
-  const TV* uinv = MODE == DS_BOTH ? (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p) : nullptr;
-  // partial sums of every chunk of the largest block when the workgroup's
-  // LDS stays within 40 KiB (several workgroups per CU), else chunks in
-  // passes of at least the first round's dense_rc * NT
-  const size_t base = dense_lds_bytes(F.inv_cap, 0, sizeof(TV), sizeof(T), MODE);
+  return D;
+}
+
+// LDS of a dense stage launch: halo values of the largest halo, and the
+// partial sums of every chunk of the largest block when the workgroup stays
+// within 40 KiB (several workgroups per CU), else chunks in passes of at
+// least the first round's dense_rc * NT
+template <typename TV, typename T, typename TY, int NT, int MODE>
+static void dense_lds_plan(const IluDev& F, int& hcap, int& part_cap, size_t& lds) {
+  const bool fwd = MODE != DS_BWD;
+  hcap = ((fwd ? F.xl_hmax : F.xu_hmax) + 3) & ~3;
+  const size_t base = dense_lds_bytes(F.inv_cap, hcap, 0, sizeof(TV), sizeof(T), MODE);
   const int need = ((fwd ? F.xl_max : F.xu_max) + 3) & ~3;
   const int first = dense_rc<TV, TY>() * NT;
-  const size_t lim = dense_wg_lds_limit();
   const size_t target = std::max<size_t>(40 * 1024, base + (size_t)first * sizeof(T));
-  int part_cap = need;
-  if (base + (size_t)need * sizeof(T) > target) part_cap = std::max<int>(first, (int)(((target - base) / sizeof(T)) & ~(size_t)3));
-  const size_t lds = dense_lds_bytes(F.inv_cap, part_cap, sizeof(TV), sizeof(T), MODE);
-  if (lds > lim) throw FsError(FS_ERR_UNSUPPORTED, "ILU block too large for the dense stage kernel");
-  auto fn = &k_ilu_stage_dense<TV, T, TB, TY, NT, G, MODE>;
+  part_cap = need;
+  if (base + (size_t)need * sizeof(T) > target)
+    part_cap = std::max<int>(first, (int)(((target - base) / sizeof(T)) & ~(size_t)3));
+  lds = dense_lds_bytes(F.inv_cap, hcap, part_cap, sizeof(TV), sizeof(T), MODE);
+}
+
+template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE, int IB>
+static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+  const int nb = b1 - b0;
+  if (nb <= 0) return;
+  const bool d = std::is_same<TV, double>::value;
+  const DenseTri<TV> D = dense_view<TV>(F, MODE != DS_BWD);
+  const TV* uinv = MODE == DS_BOTH ? (d ? (const TV*)F.uinv64.p : (const TV*)F.uinv32.p) : nullptr;
+  int hcap = 0, part_cap = 0;
+  size_t lds = 0;
+  dense_lds_plan<TV, T, TY, NT, MODE>(F, hcap, part_cap, lds);
+  if (lds > dense_wg_lds_limit()) throw FsError(FS_ERR_UNSUPPORTED, "ILU block too large for the dense stage kernel");
+  auto fn = &k_ilu_stage_dense<TV, T, TB, TY, NT, G, MODE, IB>;
   if (lds > 64 * 1024)
     hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
               "hipFuncSetAttribute");
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(NT), lds, st, D, uinv, b0, F.inv_cap, part_cap, b, yin, y);
+  hipLaunchKernelGGL(fn, dim3(nb), dim3(NT), lds, st, D, uinv, b0, F.inv_cap, hcap, part_cap, b, yin, y);
 }
 
-template <typename TV, typename T, typename TB, typename TY, int MODE>
-static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
-  // pieces of the stage's split blocks first
+template <typename TV, typename T, typename TB, typename TY, int MODE, int IB>
+static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
   const bool fwd = MODE != DS_BWD;
+  // pieces of the stage's split blocks first
   const std::vector<int>& pp = fwd ? F.xl_pc_ptr : F.xu_pc_ptr;
   int s = 0;
   while (F.sched.stage_ptr[s] != b0) ++s;
   if (pp[s + 1] > pp[s]) {
-    const bool d = std::is_same<TV, double>::value;
-    DenseTri<TV> D;
-    D.desc = fwd ? F.xldesc.as<i32x4>() : F.xudesc.as<i32x4>();
-    D.rng = fwd ? F.xlrng.as<int32_t>() : F.xurng.as<int32_t>();
-    D.xci = fwd ? F.xlci.as<int32_t>() : F.xuci.as<int32_t>();
-    D.xv = fwd ? (d ? (const TV*)F.xlv64.p : (const TV*)F.xlv32.p) : (d ? (const TV*)F.xuv64.p : (const TV*)F.xuv32.p);
-    D.inv_off = nullptr;
-    D.inv = nullptr;
-    D.split = fwd ? F.xlsplit.as<int2>() : F.xusplit.as<int2>();
-    D.part = F.dpart.p;
-    hipLaunchKernelGGL((k_ilu_ext_partials<TV, T, TY>), dim3(pp[s + 1] - pp[s]), dim3(DENSE_SPLIT_NT), 0, st, D,
+    const int hcap = ((fwd ? F.xl_hmax : F.xu_hmax) + 3) & ~3;
+    const size_t lds = sizeof(T) * ((size_t)DENSE_SPLIT + hcap);
+    auto fn = &k_ilu_ext_partials<TV, T, TY, IB>;
+    if (lds > dense_wg_lds_limit()) throw FsError(FS_ERR_UNSUPPORTED, "ILU block halo too large");
+    if (lds > 64 * 1024)
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                "hipFuncSetAttribute");
+    hipLaunchKernelGGL(fn, dim3(pp[s + 1] - pp[s]), dim3(DENSE_SPLIT_NT), lds, st, dense_view<TV>(F, fwd),
                        fwd ? F.xlpieces.as<i32x4>() : F.xupieces.as<i32x4>(), pp[s], yin);
   }
   // 128 threads for blocks of up to 64 rows (more workgroups resident per
   // CU), 512 above (larger blocks carry many more external entries: more of
   // them in flight per block); G threads per row in the in-block product
   const int mr = F.sched.max_block_rows;
-  if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 128) launch_dense<TV, T, TB, TY, 512, 4, MODE>(F, b0, b1, b, yin, y, st);
-  else launch_dense<TV, T, TB, TY, 512, 2, MODE>(F, b0, b1, b, yin, y, st);
+  if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE, IB>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE, IB>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE, IB>(F, b0, b1, b, yin, y, st);
+  else if (mr <= 128) launch_dense<TV, T, TB, TY, 512, 4, MODE, IB>(F, b0, b1, b, yin, y, st);
+  else launch_dense<TV, T, TB, TY, 512, 2, MODE, IB>(F, b0, b1, b, yin, y, st);
+}
+
+template <typename TV, typename T, typename TB, typename TY, int MODE>
+static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+  // one byte per halo index when no block's halo exceeds 256 columns
+  if ((MODE != DS_BWD ? F.xl_ib : F.xu_ib) == 1) launch_dense_ib<TV, T, TB, TY, MODE, 1>(F, b0, b1, b, yin, y, st);
+  else launch_dense_ib<TV, T, TB, TY, MODE, 2>(F, b0, b1, b, yin, y, st);
 }
 
 template <typename TV, typename T, typename TB>
@@ -615,19 +678,22 @@ template <typename TV, typename T, typename TB>
 void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
   const int ns = sched.n_stages();
   // the dense kernels keep a block's packed triangle (the fused last stage
-  // both of them) and its chunk partial sums in LDS
-  // (room for the first round's partial sums of 512 threads)
+  // both of them), its halo values and its chunk partial sums in LDS (room
+  // for the first round's partial sums of 512 threads)
   const size_t lim = dense_wg_lds_limit();
   const int pc = 512 * dense_rc<TV, T>();
-  if (dense && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
+  const int hc = (std::max(xl_hmax, xu_hmax) + 3) & ~3;
+  if (dense && dense_lds_bytes(inv_cap, hc, pc, sizeof(TV), sizeof(T), DS_FWD) <= lim) {
     // (FP32 factors only: with FP64 ones both triangles' registers cost more
     // occupancy than the launch saves)
-    const bool fuse = sizeof(TV) == 4 && fuse_last && dense_lds_bytes(inv_cap, pc, sizeof(TV), sizeof(T), DS_BOTH) <= lim;
+    const bool fuse =
+      sizeof(TV) == 4 && fuse_last && dense_lds_bytes(inv_cap, hc, pc, sizeof(TV), sizeof(T), DS_BOTH) <= lim;
     apply_dense<TV, T, TB>(*this, fuse, b, tmp, x, st);
     cnt.launches += fuse ? 2 * ns - 1 : 2 * ns;
-    // external entries (value + column), both packed triangles, the per-row
-    // chunk ranges, b read and y written in both sweeps
-    cnt.bytes += (double)n_ext * (sizeof(TV) + 4) + 2.0 * inv_total * sizeof(TV) + 8.0 * n + 4.0 * n * sizeof(T);
+    // external entries (value + halo index), the halo columns, both packed
+    // triangles, the per-row chunk ranges, b read and y written in both sweeps
+    cnt.bytes += (double)n_ext * sizeof(TV) + (double)n_xl * xl_ib + (double)n_xu * xu_ib +
+                 4.0 * (double)(n_xlh + n_xuh) + 2.0 * inv_total * sizeof(TV) + 8.0 * n + 4.0 * n * sizeof(T);
     return;
   }
   if (!std::is_same<TV, T>::value)

@@ -81,7 +81,11 @@ struct IluDev {
   int xl_max = 0, xu_max = 0;      // largest per-block count of chunks of external L / U entries
   bool fuse_last = false;          // last stage: forward and backward in one launch
   DevBuf inv_off, linv64, uinv64, linv32, uinv32;
-  DevBuf xldesc, xlrng, xlci, xlpos, xlv64, xlv32, xudesc, xurng, xuci, xupos, xuv64, xuv32;
+  DevBuf xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32;
+  DevBuf xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32;
+  int xl_ib = 1, xu_ib = 1;          // bytes per halo index of an external entry
+  int xl_hmax = 0, xu_hmax = 0;      // largest halo of a block
+  int64_t n_xlh = 0, n_xuh = 0;      // halo columns stored (padded)
   DevBuf xlsplit, xusplit, xlpieces, xupieces, dpart;  // pieces of blocks with many external chunks
   std::vector<int> xl_pc_ptr, xu_pc_ptr;               // per stage: range of pieces
   void gather_inblock(hipStream_t st);
