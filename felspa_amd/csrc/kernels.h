@@ -683,6 +683,21 @@ struct DenseTri {
   void* part;             // row partial sums of split blocks, S x rows each
 };
 
+// The update step of the inner PCG on A (alpha = rz / pq; x += alpha p, or
+// x = alpha p on the first step; r -= alpha q), carried out by the forward
+// stage kernels of the next ILU application on the rows they read r from:
+// every row belongs to exactly one stage, so the vectors are updated once.
+template <typename T>
+struct PcgFuse {
+  const double* pq;
+  const double* rz;
+  const T* p;
+  const T* q;
+  T* x;
+  T* r;  // null: no update
+  int first;
+};
+
 // 4 halo indices of one chunk: IB = 1 byte each (one word) or 2 (two words)
 template <int IB>
 struct Idx4;
@@ -873,7 +888,7 @@ __host__ __device__ inline size_t dense_lds_bytes(int inv_cap, int hcap, int par
 template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE, int IB>
 __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV* __restrict__ uinv, int blk0,
                                                         int inv_cap, int hcap, int part_cap, const TB* b,
-                                                        const TY* yin, TY* y) {
+                                                        const TY* yin, TY* y, PcgFuse<T> pf) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
   typedef TV V4 __attribute__((ext_vector_type(4)));
   typedef typename Idx4<IB>::type IX;
@@ -928,7 +943,19 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     const uint32_t rg = (uint32_t)F.rng[r0 + tid];
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
-    bi = (T)b[r0 + tid];
+    if (FWD && pf.r) {  // the PCG update of this row (k_pcg_update's arithmetic)
+      const double a = pf.pq[0] != 0.0 ? pf.rz[0] / pf.pq[0] : 0.0;
+      const T alpha = (T)a;
+      const int i = r0 + tid;
+      const T pi = pf.p[i];
+      pf.x[i] = pf.first ? alpha * pi : pf.x[i] + alpha * pi;
+      T ri = pf.r[i];
+      ri -= alpha * pf.q[i];
+      pf.r[i] = ri;
+      bi = ri;
+    } else {
+      bi = (T)b[r0 + tid];
+    }
     if (!whole) {  // partials in piece order
       const T* P = (const T*)F.part + sp.y + tid;
       for (int q = 0; q < sp.x; ++q) bi -= P[(size_t)q * nr];

@@ -605,7 +605,8 @@ static void dense_lds_plan(const IluDev& F, int& hcap, int& part_cap, size_t& ld
 }
 
 template <typename TV, typename T, typename TB, typename TY, int NT, int G, int MODE, int IB>
-static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, const PcgFuse<T>& pf,
+                         hipStream_t st) {
   const int nb = b1 - b0;
   if (nb <= 0) return;
   const bool d = std::is_same<TV, double>::value;
@@ -619,11 +620,12 @@ static void launch_dense(const IluDev& F, int b0, int b1, const TB* b, const TY*
   if (lds > 64 * 1024)
     hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
               "hipFuncSetAttribute");
-  hipLaunchKernelGGL(fn, dim3(nb), dim3(NT), lds, st, D, uinv, b0, F.inv_cap, hcap, part_cap, b, yin, y);
+  hipLaunchKernelGGL(fn, dim3(nb), dim3(NT), lds, st, D, uinv, b0, F.inv_cap, hcap, part_cap, b, yin, y, pf);
 }
 
 template <typename TV, typename T, typename TB, typename TY, int MODE, int IB>
-static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y,
+                            const PcgFuse<T>& pf, hipStream_t st) {
   const bool fwd = MODE != DS_BWD;
   // pieces of the stage's split blocks first
   const std::vector<int>& pp = fwd ? F.xl_pc_ptr : F.xu_pc_ptr;
@@ -644,38 +646,44 @@ static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const 
   // CU), 512 above (larger blocks carry many more external entries: more of
   // them in flight per block); G threads per row in the in-block product
   const int mr = F.sched.max_block_rows;
-  if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE, IB>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE, IB>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE, IB>(F, b0, b1, b, yin, y, st);
-  else if (mr <= 128) launch_dense<TV, T, TB, TY, 512, 4, MODE, IB>(F, b0, b1, b, yin, y, st);
-  else launch_dense<TV, T, TB, TY, 512, 2, MODE, IB>(F, b0, b1, b, yin, y, st);
+  if (mr <= 16) launch_dense<TV, T, TB, TY, 128, 8, MODE, IB>(F, b0, b1, b, yin, y, pf, st);
+  else if (mr <= 32) launch_dense<TV, T, TB, TY, 128, 4, MODE, IB>(F, b0, b1, b, yin, y, pf, st);
+  else if (mr <= 64) launch_dense<TV, T, TB, TY, 128, 2, MODE, IB>(F, b0, b1, b, yin, y, pf, st);
+  else if (mr <= 128) launch_dense<TV, T, TB, TY, 512, 4, MODE, IB>(F, b0, b1, b, yin, y, pf, st);
+  else launch_dense<TV, T, TB, TY, 512, 2, MODE, IB>(F, b0, b1, b, yin, y, pf, st);
 }
 
 template <typename TV, typename T, typename TB, typename TY, int MODE>
-static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y, hipStream_t st) {
+static void launch_dense_g(const IluDev& F, int b0, int b1, const TB* b, const TY* yin, TY* y,
+                           const PcgFuse<T>& pf, hipStream_t st) {
   // one byte per halo index when no block's halo exceeds 256 columns
-  if ((MODE != DS_BWD ? F.xl_ib : F.xu_ib) == 1) launch_dense_ib<TV, T, TB, TY, MODE, 1>(F, b0, b1, b, yin, y, st);
-  else launch_dense_ib<TV, T, TB, TY, MODE, 2>(F, b0, b1, b, yin, y, st);
+  if ((MODE != DS_BWD ? F.xl_ib : F.xu_ib) == 1) launch_dense_ib<TV, T, TB, TY, MODE, 1>(F, b0, b1, b, yin, y, pf, st);
+  else launch_dense_ib<TV, T, TB, TY, MODE, 2>(F, b0, b1, b, yin, y, pf, st);
 }
 
 template <typename TV, typename T, typename TB>
-static void apply_dense(const IluDev& F, bool fuse, const TB* b, T* tmp, T* x, hipStream_t st) {
+static void apply_dense(const IluDev& F, bool fuse, const TB* b, T* tmp, T* x, const PcgFuse<T>& pf,
+                        hipStream_t st) {
   const Schedule& sc = F.sched;
   const int ns = sc.n_stages();
   const int nf = fuse ? ns - 1 : ns;  // stages with a forward launch of their own
+  const PcgFuse<T> none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
   for (int s = 0; s < nf; ++s)
-    launch_dense_g<TV, T, TB, T, DS_FWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], b, tmp, tmp, st);
+    launch_dense_g<TV, T, TB, T, DS_FWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], b, tmp, tmp, pf, st);
   if constexpr (sizeof(TV) == 4) {
-    if (fuse) launch_dense_g<TV, T, TB, T, DS_BOTH>(F, sc.stage_ptr[ns - 1], sc.stage_ptr[ns], b, tmp, x, st);
+    if (fuse) launch_dense_g<TV, T, TB, T, DS_BOTH>(F, sc.stage_ptr[ns - 1], sc.stage_ptr[ns], b, tmp, x, pf, st);
   }
   for (int s = nf - 1; s >= 0; --s)
-    launch_dense_g<TV, T, T, T, DS_BWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], tmp, x, x, st);
+    launch_dense_g<TV, T, T, T, DS_BWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], tmp, x, x, none, st);
 }
 
 // x = (LU)^{-1} b with TV factor values and T vectors / arithmetic: forward
-// into tmp, backward into x (tmp may alias x)
+// into tmp, backward into x (tmp may alias x). With pf, the PCG update step
+// that produces b (= pf->r) runs first: inside the forward stage kernels, or
+// as its own kernel ahead of the substitution kernels.
 template <typename TV, typename T, typename TB>
-void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const {
+void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt, const PcgFuse<T>* pf) const {
+  const PcgFuse<T> none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
   const int ns = sched.n_stages();
   // the dense kernels keep a block's packed triangle (the fused last stage
   // both of them), its halo values and its chunk partial sums in LDS (room
@@ -688,7 +696,7 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) con
     // occupancy than the launch saves)
     const bool fuse =
       sizeof(TV) == 4 && fuse_last && dense_lds_bytes(inv_cap, hc, pc, sizeof(TV), sizeof(T), DS_BOTH) <= lim;
-    apply_dense<TV, T, TB>(*this, fuse, b, tmp, x, st);
+    apply_dense<TV, T, TB>(*this, fuse, b, tmp, x, pf ? *pf : none, st);
     cnt.launches += fuse ? 2 * ns - 1 : 2 * ns;
     // external entries (value + halo index), the halo columns, both packed
     // triangles, the per-row chunk ranges, b read and y written in both sweeps
@@ -698,6 +706,11 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) con
   }
   if (!std::is_same<TV, T>::value)
     throw FsError(FS_ERR_UNSUPPORTED, "ILU application with FP32 factors and FP64 vectors needs the dense kernels");
+  if (pf && pf->r) {
+    hipLaunchKernelGGL((k_pcg_update<T>), dim3(256), dim3(BLOCK), 0, st, (int64_t)n, pf->pq, pf->rz, pf->p, pf->q, pf->x,
+                       pf->r, pf->first);
+    cnt.launches++;
+  }
   using TS = T;  // substitution kernels: factor values and vectors of one type
   for (int s = 0; s < ns; ++s)
     launch_stage_g<TS, TB, TS, true, true>(*this, sched.stage_ptr[s], sched.stage_ptr[s + 1], b, tmp, st);
@@ -706,10 +719,10 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) con
   cnt.launches += 2 * ns;
   cnt.bytes += (double)nnz * (sizeof(T) + 4) + 16.0 * (n + 1) + 3.0 * n * sizeof(T) + 20.0 * n;
 }
-template void IluDev::apply<double, double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
-template void IluDev::apply<float, double, double>(const double*, double*, double*, hipStream_t, Counters&) const;
-template void IluDev::apply<float, float, double>(const double*, float*, float*, hipStream_t, Counters&) const;
-template void IluDev::apply<float, float, float>(const float*, float*, float*, hipStream_t, Counters&) const;
+template void IluDev::apply<double, double, double>(const double*, double*, double*, hipStream_t, Counters&, const PcgFuse<double>*) const;
+template void IluDev::apply<float, double, double>(const double*, double*, double*, hipStream_t, Counters&, const PcgFuse<double>*) const;
+template void IluDev::apply<float, float, double>(const double*, float*, float*, hipStream_t, Counters&, const PcgFuse<float>*) const;
+template void IluDev::apply<float, float, float>(const float*, float*, float*, hipStream_t, Counters&, const PcgFuse<float>*) const;
 
 // transposed solve with a transposed factor (lower part = U^T with diagonal,
 // upper part = L^T unit)

@@ -95,7 +95,7 @@ struct IluDev {
   void factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit, hipStream_t st,
                  int* d_err, Counters& cnt);
   template <typename TV, typename T, typename TB>
-  void apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;
+  void apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt, const fsk::PcgFuse<T>* pf = nullptr) const;
   template <typename T>
   void apply_transposed(const T* b, T* tmp, T* x, hipStream_t st, Counters& cnt) const;
 };

@@ -305,12 +305,18 @@ struct fs_handle {
     for (int it = 0; it < iters; ++it) {
       launch_spmv_dot<TV, T>(pv, q, part_pq);
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
-      hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
-                         it == 0 ? 1 : 0);
-      cnt.launches += 3;
+      cnt.launches += 2;
       cnt.bytes += Ablk.nnz * (sizeof(TV) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
-      if (it + 1 == iters) break;
-      FA.apply<TV, T, T>(r, y, z, st, cnt);
+      if (it + 1 == iters) {
+        hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
+                           it == 0 ? 1 : 0);
+        cnt.launches++;
+        break;
+      }
+      // alpha = rz / pq, x += alpha p, r -= alpha q: done by the forward
+      // stages of the ILU application that reads r
+      const PcgFuse<T> pf{S(SC_PCG), rz + cur, pv, q, x, r, it == 0 ? 1 : 0};
+      FA.apply<TV, T, T>(r, y, z, st, cnt, &pf);
       hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
       hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
                          rz + (cur ^ 1), z, pv, 0);
