@@ -216,6 +216,18 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off,
   }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i mod 8), and each XCD has its own L2. Renumbering workgroup i as
+// (i mod 8) * (n / 8) + i / 8 gives every XCD one contiguous range of the
+// work (rows, blocks), so the values it gathers from its neighbourhood stay
+// in its own L2; the tail beyond 8 (n / 8) keeps its numbers. (Used by the
+// ILU stages, 5 % faster at C2; the A-block SpMV measured slower with it.)
+__device__ __forceinline__ int xcd_contiguous(int i, int n) {
+  constexpr int NX = 8;
+  const int per = n / NX;
+  return i < per * NX ? (i % NX) * per + i / NX : i;
+}
+
 // q = A p on a CSR4 matrix with TV values and T vectors / arithmetic;
 // partial[b] = sum over the workgroup's rows of p_i q_i
 template <typename TV, typename T, int G>
@@ -226,7 +238,8 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
                                                      double* __restrict__ partial) {
   __shared__ double sh[BLOCK / WAVE];
   const int lane = threadIdx.x % G;
-  const int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G;
+  const int wg = blockIdx.x;
+  const int64_t row = (int64_t)wg * (BLOCK / G) + threadIdx.x / G;
   double contrib = 0;
   if (row < nrows) {
     const T acc = row_dot4x2<TV, T, T, G>(ci4, val4, p, rp4[row], rp4[row + 1], lane);
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
     }
   }
   contrib = block_sum(contrib, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = contrib;
+  if (threadIdx.x == 0) partial[wg] = contrib;
 }
 
 // CSR SpMV: G lanes per row, BLOCK/G rows per workgroup.
@@ -904,7 +917,7 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   T* t = part + part_cap;
   T* t2 = t + 256;
   const int tid = threadIdx.x;
-  const int blk = blk0 + blockIdx.x;
+  const int blk = blk0 + xcd_contiguous(blockIdx.x, gridDim.x);
   const i32x4 d = F.desc[blk];
   const int2 sp = F.split[blk];
   const int2 hl = F.halo[blk];
