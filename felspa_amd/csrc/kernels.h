@@ -605,7 +605,10 @@ __device__ void ilu_fwd_block(const IluView<T>& F, int blk, const TB* __restrict
       L.ec[k] = F.lrow[e0 + k];
     }
   if (NT > WAVE) __syncthreads(); else wave_sync();
-  if (tid < WAVE) {
+  if (ne == 0) {  // no in-block entries (point-coloured blocks): rows independent
+    if (!UNIT)
+      for (int i = tid; i < nr; i += NT) L.s[i] /= L.d[i];
+  } else if (tid < WAVE) {
     if (in_lds) push_phase<T, UNIT, true>(L, L.ev, L.ec, nr, tid);
     else push_phase<T, UNIT, true>(L, F.lv + e0, F.lrow + e0, nr, tid);
   }
@@ -642,7 +645,9 @@ __device__ void ilu_bwd_block(const IluView<T>& F, int blk, const TB* __restrict
       L.ec[k] = F.urow[e0 + k];
     }
   if (NT > WAVE) __syncthreads(); else wave_sync();
-  if (tid < WAVE) {
+  if (ne == 0) {  // no in-block entries (point-coloured blocks): rows independent
+    for (int i = tid; i < nr; i += NT) L.s[i] /= L.d[i];
+  } else if (tid < WAVE) {
     if (in_lds) push_phase<T, UNIT, false>(L, L.ev, L.ec, nr, tid);
     else push_phase<T, UNIT, false>(L, F.uv + e0, F.urow + e0, nr, tid);
   }
