@@ -252,6 +252,106 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
   if (threadIdx.x == 0) partial[wg] = contrib;
 }
 
+// q = A p and per-block partials of p.q on the A block stored by lattice
+// blocks (the ILU's blocks): each block's entries reference the block's own
+// column list (its distinct columns, ascending) through 1- or 2-byte indices,
+// rows padded to chunks of 4. One workgroup per block: the chunks (16-byte
+// value vectors + index words) and the column list load in one round, the
+// block's p values are gathered once into LDS, chunk sums go to LDS and
+// every row sums its chunks in order (two partial sums by chunk parity).
+template <int IB>
+struct Idx4;
+template <typename TV, typename T, int IB, int NT>
+__global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ desc, const int2* __restrict__ cinfo,
+                                                     const int32_t* __restrict__ rng, const void* __restrict__ xidx,
+                                                     const TV* __restrict__ xv, const int32_t* __restrict__ cols,
+                                                     const T* __restrict__ p, T* __restrict__ q,
+                                                     double* __restrict__ partial, int ccap) {
+  extern __shared__ __attribute__((aligned(16))) char dsm[];
+  __shared__ double sh[NT / WAVE];
+  typedef TV V4 __attribute__((ext_vector_type(4)));
+  typedef typename Idx4<IB>::type IX;
+  constexpr int R = 4;
+  T* xs = (T*)dsm;
+  T* part = xs + ccap;
+  const int tid = threadIdx.x;
+  const int blk = blockIdx.x;
+  const i32x4 d = desc[blk];
+  const int2 ci = cinfo[blk];
+  const int r0 = d.x, nr = d.y, nc = d.w;
+  const IX* isrc = (const IX*)xidx + d.z;
+  const V4* vsrc = (const V4*)xv + d.z;
+  IX cx[R];
+  V4 vv[R];
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int c = u * NT + tid;
+    const int cc = c < nc ? c : 0;
+    cx[u] = __builtin_nontemporal_load(isrc + cc);
+    vv[u] = __builtin_nontemporal_load(vsrc + cc);
+  }
+  int cs = 0, cn = 0;
+  if (tid < nr) {
+    const uint32_t rg = (uint32_t)rng[r0 + tid];
+    cs = (int)(rg & 0xffffu);
+    cn = (int)(rg >> 16);
+  }
+  const i32x4* c4 = (const i32x4*)(cols + ci.x);
+  for (int k = tid; 4 * k < ci.y; k += NT) {
+    const i32x4 c = __builtin_nontemporal_load(c4 + k);
+    const T a0 = p[c.x], a1 = p[c.y], a2 = p[c.z], a3 = p[c.w];
+    xs[4 * k] = a0;
+    xs[4 * k + 1] = a1;
+    xs[4 * k + 2] = a2;
+    xs[4 * k + 3] = a3;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < R; ++u) {
+    const int c = u * NT + tid;
+    if (c < nc) {
+      T s = (T)vv[u].x * xs[Idx4<IB>::get(cx[u], 0)];
+      s += (T)vv[u].y * xs[Idx4<IB>::get(cx[u], 1)];
+      s += (T)vv[u].z * xs[Idx4<IB>::get(cx[u], 2)];
+      s += (T)vv[u].w * xs[Idx4<IB>::get(cx[u], 3)];
+      part[c] = s;
+    }
+  }
+  for (int c0 = R * NT + tid; c0 < nc; c0 += R * NT) {  // blocks beyond one round
+    IX xr[R];
+    V4 vr[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int c = c0 + u * NT < nc ? c0 + u * NT : c0;
+      xr[u] = __builtin_nontemporal_load(isrc + c);
+      vr[u] = __builtin_nontemporal_load(vsrc + c);
+    }
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int c = c0 + u * NT < nc ? c0 + u * NT : c0;
+      T s = (T)vr[u].x * xs[Idx4<IB>::get(xr[u], 0)];
+      s += (T)vr[u].y * xs[Idx4<IB>::get(xr[u], 1)];
+      s += (T)vr[u].z * xs[Idx4<IB>::get(xr[u], 2)];
+      s += (T)vr[u].w * xs[Idx4<IB>::get(xr[u], 3)];
+      part[c] = s;
+    }
+  }
+  __syncthreads();
+  double contrib = 0;
+  if (tid < nr) {
+    T s0 = 0, s1 = 0;
+    for (int k = 0; k < cn; ++k) {
+      if (k & 1) s1 += part[cs + k];
+      else s0 += part[cs + k];
+    }
+    const T acc = s0 + s1;
+    q[r0 + tid] = acc;
+    contrib = (double)acc * (double)p[r0 + tid];
+  }
+  contrib = block_sum(contrib, sh);
+  if (tid == 0) partial[blk] = contrib;
+}
+
 // CSR SpMV: G lanes per row, BLOCK/G rows per workgroup.
 //   MODE 0: y = M x
 //   MODE 1: y = b - M x

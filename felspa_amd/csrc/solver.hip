@@ -799,6 +799,99 @@ void DevCsr::upload_pattern(const HostCsr& h, bool with_padded) {
   G4 = g > 64 ? 32 : g > 32 ? 16 : g > 12 ? 8 : 4;
 }
 
+void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
+  n = h.n_rows;
+  nnz = h.nnz();
+  nb = (int)blk_ptr.size() - 1;
+  std::vector<int32_t> dsc(4 * (size_t)nb), ci(2 * (size_t)nb), rg(std::max<int64_t>(1, n));
+  int64_t q = 0;
+  cnmax = rmax = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int32_t r0 = blk_ptr[b], r1 = blk_ptr[b + 1];
+    const int64_t qb = q;
+    for (int32_t r = r0; r < r1; ++r) {
+      const int64_t nc = (h.rp[r + 1] - h.rp[r] + 3) / 4;
+      if (q - qb > 65535 || nc > 65535) throw FsError(FS_ERR_UNSUPPORTED, "matrix block with too many entries");
+      rg[r] = (int32_t)((q - qb) | (nc << 16));
+      q += nc;
+    }
+    if (q > INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "matrix with too many entries");
+    dsc[4 * b] = r0;
+    dsc[4 * b + 1] = r1 - r0;
+    dsc[4 * b + 2] = (int32_t)qb;
+    dsc[4 * b + 3] = (int32_t)(q - qb);
+    cnmax = std::max<int>(cnmax, (int)(q - qb));
+    rmax = std::max<int>(rmax, r1 - r0);
+  }
+  n_ent = 4 * q;
+  // the blocks' column lists and every entry's index into its block's list
+  std::vector<int32_t> cl, hb, loc(std::max<int64_t>(4, n_ent), 0), stamp(std::max<int64_t>(1, h.n_cols), -1),
+    slot(std::max<int64_t>(1, h.n_cols), 0);
+  pos.assign(nnz, 0);
+  cmax = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int32_t r0 = blk_ptr[b], r1 = blk_ptr[b + 1];
+    hb.clear();
+    for (int64_t e = h.rp[r0]; e < h.rp[r1]; ++e)
+      if (stamp[h.ci[e]] != b) {
+        stamp[h.ci[e]] = b;
+        hb.push_back(h.ci[e]);
+      }
+    std::sort(hb.begin(), hb.end());
+    if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "matrix block with too many columns");
+    for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
+    int64_t o = 4 * (int64_t)dsc[4 * b + 2];
+    for (int32_t r = r0; r < r1; ++r) {
+      const int64_t o0 = o;
+      for (int64_t e = h.rp[r]; e < h.rp[r + 1]; ++e, ++o) {
+        loc[o] = slot[h.ci[e]];
+        pos[e] = o;
+      }
+      for (; (o - o0) % 4; ++o) loc[o] = 0;
+    }
+    ci[2 * b] = (int32_t)cl.size();
+    ci[2 * b + 1] = (int32_t)hb.size();
+    cmax = std::max<int>(cmax, (int)hb.size());
+    cl.insert(cl.end(), hb.begin(), hb.end());
+    while (cl.size() % 4) cl.push_back(hb.empty() ? 0 : hb[0]);
+  }
+  if (cl.empty()) cl.assign(4, 0);
+  if (cl.size() > (size_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "matrix with too many block columns");
+  // one row per thread of a 512-thread workgroup, the block's column values
+  // and chunk sums in LDS (FP64 at most): otherwise the caller keeps CSR4
+  if (rmax > 512 || 8.0 * (((cmax + 3) & ~3) + (double)cnmax) > 160.0 * 1024) {
+    nb = 0;
+    return;
+  }
+  n_cols = (int64_t)cl.size();
+  ib = cmax <= 256 ? 1 : 2;
+  if (ib == 1) {
+    std::vector<uint8_t> ix(loc.size());
+    for (size_t e = 0; e < loc.size(); ++e) ix[e] = (uint8_t)loc[e];
+    idx.upload(ix.data(), ix.size());
+  } else {
+    std::vector<uint16_t> ix(loc.size());
+    for (size_t e = 0; e < loc.size(); ++e) ix[e] = (uint16_t)loc[e];
+    idx.upload(ix.data(), 2 * ix.size());
+  }
+  desc.upload(dsc.data(), 4 * dsc.size());
+  cinfo.upload(ci.data(), 4 * ci.size());
+  rng.upload(rg.data(), 4 * rg.size());
+  cols.upload(cl.data(), 4 * cl.size());
+  pad.assign(std::max<int64_t>(4, n_ent), 0.0);
+  v64.alloc(8 * pad.size());
+  v32.alloc(4 * pad.size());
+}
+
+void BlkCsr::upload_values(const double* vals, hipStream_t st) {
+  if (nb == 0) return;
+  for (int64_t e = 0; e < nnz; ++e) pad[pos[e]] = vals[e];
+  v64.upload(pad.data(), 8 * pad.size());
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad.size())), dim3(BLOCK), 0, st,
+                     (int64_t)pad.size(), v64.as<double>(), v32.as<float>());
+  hip_check(hipStreamSynchronize(st), "sync");
+}
+
 void DevCsr::upload_values(const double* vals, hipStream_t st) {
   v64.upload(vals, 8 * std::max<int64_t>(1, nnz));
   if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));

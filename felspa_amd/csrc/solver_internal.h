@@ -61,6 +61,20 @@ struct DevCsr {
   void upload_values(const double* vals, hipStream_t st);
 };
 
+// A CSR matrix stored by row blocks for the blocked SpMV (k_spmv_blk_dot):
+// every block's entries reference the block's own column list (its distinct
+// columns, ascending) through 1-byte indices (2 when a list exceeds 256
+// columns); rows padded to chunks of 4 entries (value 0, index 0).
+struct BlkCsr {
+  int64_t n = 0, nnz = 0, n_ent = 0, n_cols = 0;  // rows, entries, padded entries, column-list entries
+  int nb = 0, ib = 1, cmax = 0, cnmax = 0, rmax = 0;
+  DevBuf desc, cinfo, rng, idx, cols, v64, v32;
+  std::vector<int64_t> pos;  // entry -> padded position
+  std::vector<double> pad;   // host staging of the padded values
+  void build(const HostCsr& h, const std::vector<int32_t>& blk_ptr);
+  void upload_values(const double* vals, hipStream_t st);
+};
+
 struct IluDev {
   int64_t n = 0, nnz = 0;
   Schedule sched;

@@ -171,6 +171,7 @@ struct fs_handle {
   int64_t nu = 0, np = 0, n = 0, ld = 0;
   hipStream_t st = nullptr;
   DevCsr K, Mp, Ablk;                                     // K, M_p and A alone, solver order
+  BlkCsr Ablb;                                            // A by the ILU's lattice blocks (inner PCG SpMV)
   std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp;  // caller entry -> K / Mp position
   std::vector<int64_t> ablk_from_k;                       // A entry -> K position
   std::vector<double> hK, hMp, hA;                        // host value staging
@@ -306,7 +307,7 @@ struct fs_handle {
       launch_spmv_dot<TV, T>(pv, q, part_pq);
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
       cnt.launches += 2;
-      cnt.bytes += Ablk.nnz * (sizeof(TV) + 4.0) + 8.0 * (nu + 1) + sizeof(T) * 7.0 * nu;
+      cnt.bytes += ablk_spmv_bytes(sizeof(TV), sizeof(T)) + sizeof(T) * 5.0 * nu;
       if (it + 1 == iters) {
         hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
                            it == 0 ? 1 : 0);
@@ -326,12 +327,45 @@ struct fs_handle {
     }
   }
 
-  int spmv_dot_grid() const { return (int)std::max<int64_t>(1, (nu * Ablk.G4 + BLOCK - 1) / BLOCK); }
+  int spmv_dot_grid() const {
+    return Ablb.nb > 0 ? Ablb.nb : (int)std::max<int64_t>(1, (nu * Ablk.G4 + BLOCK - 1) / BLOCK);
+  }
+
+  // bytes of one A-block SpMV with s_v-byte values and s_x-byte vectors:
+  // the blocked layout (values + indices, column lists, row ranges, p read
+  // and q written) or the CSR4 one
+  double ablk_spmv_bytes(double sv, double sx) const {
+    if (Ablb.nb > 0)
+      return (double)Ablb.nnz * (sv + Ablb.ib) + 4.0 * (double)Ablb.n_cols + 4.0 * nu + 2.0 * nu * sx;
+    return Ablk.nnz * (sv + 4.0) + 8.0 * (nu + 1) + sx * 2.0 * nu;
+  }
 
   // q = A p and the per-workgroup partials of p.q, on the padded A block;
   // TV selects the stored values (Ablk.v4_32 or Ablk.v4_64), T the vectors
   template <typename TV, typename T>
   void launch_spmv_dot(const T* pv, T* q, double* part) {
+    if (Ablb.nb > 0) {  // by lattice blocks: 128 threads for up to 64 rows, 512 above
+      const bool d = std::is_same<TV, double>::value;
+      const TV* v = d ? (const TV*)Ablb.v64.p : (const TV*)Ablb.v32.p;
+      const int ccap = (Ablb.cmax + 3) & ~3;
+      const size_t lds = sizeof(T) * ((size_t)ccap + Ablb.cnmax);
+#define FS_SB(IB, NT)                                                                                          \
+  {                                                                                                            \
+    auto fn = &k_spmv_blk_dot<TV, T, IB, NT>;                                                                  \
+    if (lds > 64 * 1024)                                                                                       \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(Ablb.nb), dim3(NT), lds, st, Ablb.desc.as<i32x4>(), Ablb.cinfo.as<int2>(),     \
+                       Ablb.rng.as<int32_t>(), Ablb.idx.p, v, Ablb.cols.as<int32_t>(), pv, q, part, ccap);     \
+  }
+      if (Ablb.rmax <= 64) {
+        if (Ablb.ib == 1) FS_SB(1, 128) else FS_SB(2, 128)
+      } else {
+        if (Ablb.ib == 1) FS_SB(1, 512) else FS_SB(2, 512)
+      }
+#undef FS_SB
+      return;
+    }
     const int g = spmv_dot_grid();
     using V4 = typename Vec4<TV>::type;
     const V4* av = std::is_same<TV, double>::value ? (const V4*)Ablk.v4_64.p : (const V4*)Ablk.v4_32.p;
@@ -498,6 +532,7 @@ struct fs_handle {
     K.upload_values(hK.data(), st);
     for (size_t e = 0; e < ablk_from_k.size(); ++e) hA[e] = hK[ablk_from_k[e]];
     Ablk.upload_values(hA.data(), st);
+    Ablb.upload_values(hA.data(), st);
     Mp.upload_values(hMp.data(), st);
     dirty = true;
   }
@@ -684,7 +719,7 @@ struct fs_handle {
       hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
       hipLaunchKernelGGL((k_pcg_update<double>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, xv, rr, 0);
       cnt.launches += 3;
-      cnt.bytes += Ablk.nnz * 12.0 + 8.0 * (nu + 1) + 8.0 * 7.0 * nu;
+      cnt.bytes += ablk_spmv_bytes(8.0, 8.0) + 8.0 * 5.0 * nu;
       norm(rr, nu, SC_TMP + 2);
       const double res = read_scalar(SC_TMP + 2);
       stt = c.check(it, res);
@@ -853,6 +888,7 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
       if (Kh.ci[q] < nu) { Ah.ci[o] = Kh.ci[q]; h.ablk_from_k[o] = q; ++o; }
   }
   h.Ablk.upload_pattern(Ah, true);
+  h.Ablb.build(Ah, h.FA.sched.blk_ptr);
   h.hA.assign(Ah.nnz(), 0.0);
 }
 
@@ -1313,7 +1349,7 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
         else
           h->launch_spmv_dot<double, double>(h->vin.as<double>(), (double*)base, dptr<double>(h->pcg_part));
         const double sv = (mixed || f32) ? 4.0 : 8.0, sx = f32 ? 4.0 : 8.0;
-        h->cnt.bytes = h->Ablk.nnz * (sv + 4.0) + 8.0 * (nu + 1) + sx * 2.0 * nu;
+        h->cnt.bytes = h->ablk_spmv_bytes(sv, sx);
         break;
       }
       case 1:
