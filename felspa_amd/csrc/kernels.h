@@ -885,8 +885,9 @@ __device__ __forceinline__ T chunk_dot(const V4& v, typename Idx4<IB>::type ix, 
 }
 
 // External sums of one piece (chunks [ca, ce) of block blk, ce - ca <=
-// DENSE_SPLIT) of a block with many external entries: the block's halo is
-// gathered into LDS, then every row's sum over its chunks inside the piece,
+// DENSE_SPLIT) of a block with many external entries: the piece's halo (its
+// entries' distinct columns; their indices point into it) is gathered into
+// LDS, then every row's sum over its chunks inside the piece,
 // in chunk order, goes to part[first + piece rows + row]. One workgroup per
 // piece, all chunk loads in one round.
 template <typename TV, typename T, typename TY, int IB>
@@ -900,10 +901,11 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
   T* part = (T*)dsm;
   T* hv = part + DENSE_SPLIT;
   const int tid = threadIdx.x;
-  const i32x4 pc = pieces[pc0 + blockIdx.x];  // {block, first chunk, end chunk, piece}
+  const i32x4 pc = pieces[2 * (pc0 + blockIdx.x)];      // {block, first chunk, end chunk, piece}
+  const i32x4 ph = pieces[2 * (pc0 + blockIdx.x) + 1];  // {the piece's halo, halo columns, 0, 0}
   const int blk = pc.x, ca = pc.y, ce = pc.z, piece = pc.w;
   const i32x4 d = F.desc[blk];
-  const int2 hl = F.halo[blk];
+  const int2 hl = make_int2(ph.x, ph.y);
   const int r0 = d.x, nr = d.y;
   const IX* isrc = (const IX*)F.xidx + d.z;
   const V4* vsrc = (const V4*)F.xv + d.z;

@@ -308,7 +308,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     // and its chunk count (high 16 bits).
     auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& didx, DevBuf& dhcol, DevBuf& dhalo, DevBuf& dpos,
                    DevBuf& dv64, DevBuf& dv32, int64_t& n_ent, int& c_max, std::vector<int32_t>& dsc, int& ib,
-                   int& h_max, int64_t& n_halo) {
+                   int& h_max, int& ph_max, int64_t& n_halo, std::vector<std::vector<int32_t>>& phl) {
       const int nb = sched.n_blocks();
       std::vector<int32_t> rg(n);
       dsc.assign(4 * (size_t)nb, 0);
@@ -349,32 +349,50 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       // the block's halo: its distinct external columns, ascending (nearby
       // lanes gather nearby values); every entry stores its column's index in
       // the halo (1 byte when no halo exceeds 256 columns, else 2); halos
-      // start at multiples of 4
-      std::vector<int32_t> hc, hl(2 * (size_t)nb), hb;
+      // start at multiples of 4. A block split into pieces (more than 2
+      // DENSE_SPLIT chunks) has a halo per piece instead, listed in phl.
+      std::vector<int32_t> hc, hl(2 * (size_t)nb, 0), hb;
       std::vector<int32_t> loc(cc.size(), 0), stamp(n, -1), slot(n, 0);
+      phl.assign(nb, {});
       h_max = 0;
-      for (int b = 0; b < nb; ++b) {
-        const int64_t a = 4 * (int64_t)dsc[4 * b + 2], z = a + 4 * (int64_t)dsc[4 * b + 3];
+      int stamp_id = 0;
+      auto halo = [&](int64_t a, int64_t z, int32_t& off, int32_t& cnt) {
         hb.clear();
+        ++stamp_id;
         for (int64_t e2 = a; e2 < z; ++e2)
-          if (stamp[cc[e2]] != b) {
-            stamp[cc[e2]] = b;
+          if (stamp[cc[e2]] != stamp_id) {
+            stamp[cc[e2]] = stamp_id;
             hb.push_back(cc[e2]);
           }
         std::sort(hb.begin(), hb.end());
         for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
         for (int64_t e2 = a; e2 < z; ++e2) loc[e2] = slot[cc[e2]];
         if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external columns");
-        hl[2 * b] = (int32_t)hc.size();
-        hl[2 * b + 1] = (int32_t)hb.size();
-        h_max = std::max<int>(h_max, (int)hb.size());
+        off = (int32_t)hc.size();
+        cnt = (int32_t)hb.size();
         hc.insert(hc.end(), hb.begin(), hb.end());
         while (hc.size() % 4) hc.push_back(hb.empty() ? 0 : hb[0]);
+      };
+      for (int b = 0; b < nb; ++b) {
+        const int nc = dsc[4 * b + 3];
+        const int64_t a = 4 * (int64_t)dsc[4 * b + 2], z = a + 4 * (int64_t)nc;
+        if (nc > 2 * DENSE_SPLIT) {
+          for (int c = 0; c < nc; c += DENSE_SPLIT) {
+            int32_t off = 0, cnt = 0;
+            halo(a + 4 * (int64_t)c, a + 4 * (int64_t)std::min(nc, c + DENSE_SPLIT), off, cnt);
+            phl[b].push_back(off);
+            phl[b].push_back(cnt);
+            ph_max = std::max<int>(ph_max, cnt);
+          }
+        } else {
+          halo(a, z, hl[2 * b], hl[2 * b + 1]);
+          h_max = std::max<int>(h_max, hl[2 * b + 1]);
+        }
       }
       if (hc.size() > (size_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external columns");
       if (hc.empty()) hc.assign(4, 0);
       n_halo = (int64_t)hc.size();
-      ib = h_max <= 256 ? 1 : 2;
+      ib = std::max(h_max, ph_max) <= 256 ? 1 : 2;
       if (ib == 1) {
         std::vector<uint8_t> ix(loc.size());
         for (size_t e2 = 0; e2 < loc.size(); ++e2) ix[e2] = (uint8_t)loc[e2];
@@ -394,12 +412,17 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     };
     n_ext = sum_lext + sum_uext;
     std::vector<int32_t> dl, du;
-    ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, n_xlh);
-    ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, n_xuh);
+    std::vector<std::vector<int32_t>> pl, pu;
+    xl_phmax = xu_phmax = 0;
+    ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, xl_phmax,
+        n_xlh, pl);
+    ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, xu_phmax,
+        n_xuh, pu);
     // blocks with many external chunks are summed in pieces of DENSE_SPLIT
     // chunks by workgroups of their own; pieces listed stage by stage
     int64_t part_need = 0;
-    auto split = [&](const std::vector<int32_t>& dsc, DevBuf& dsplit, DevBuf& dpieces, std::vector<int>& pc_ptr) {
+    auto split = [&](const std::vector<int32_t>& dsc, const std::vector<std::vector<int32_t>>& phl, DevBuf& dsplit,
+                     DevBuf& dpieces, std::vector<int>& pc_ptr) {
       const int nb = sched.n_blocks();
       std::vector<int32_t> sp(2 * (size_t)nb), pcs;
       pc_ptr.assign(1, 0);
@@ -411,25 +434,29 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
           sp[2 * b] = S;
           sp[2 * b + 1] = (int32_t)off;
           if (S > 1) {
-            for (int q = 0; q < S; ++q) {
+            for (int q = 0; q < S; ++q) {  // {block, first chunk, end chunk, piece}, {halo, halo columns, 0, 0}
               pcs.push_back(b);
               pcs.push_back(q * DENSE_SPLIT);
               pcs.push_back(std::min(nc, (q + 1) * DENSE_SPLIT));
               pcs.push_back(q);
+              pcs.push_back(phl[b][2 * q]);
+              pcs.push_back(phl[b][2 * q + 1]);
+              pcs.push_back(0);
+              pcs.push_back(0);
             }
             off += (int64_t)S * nr;
             if (off > INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external entries");
           }
         }
-        pc_ptr.push_back((int)(pcs.size() / 4));
+        pc_ptr.push_back((int)(pcs.size() / 8));
       }
-      if (pcs.empty()) pcs.assign(4, 0);
+      if (pcs.empty()) pcs.assign(8, 0);
       dsplit.upload(sp.data(), 4 * sp.size());
       dpieces.upload(pcs.data(), 4 * pcs.size());
       part_need = std::max(part_need, off);
     };
-    split(dl, xlsplit, xlpieces, xl_pc_ptr);
-    split(du, xusplit, xupieces, xu_pc_ptr);
+    split(dl, pl, xlsplit, xlpieces, xl_pc_ptr);
+    split(du, pu, xusplit, xupieces, xu_pc_ptr);
     dpart.alloc(8 * std::max<int64_t>(1, part_need));
     // the last stage has no external U entries when no two blocks of a stage
     // couple: its backward solve is then in-block only and runs in the same
@@ -632,7 +659,7 @@ static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const 
   int s = 0;
   while (F.sched.stage_ptr[s] != b0) ++s;
   if (pp[s + 1] > pp[s]) {
-    const int hcap = ((fwd ? F.xl_hmax : F.xu_hmax) + 3) & ~3;
+    const int hcap = ((fwd ? F.xl_phmax : F.xu_phmax) + 3) & ~3;
     const size_t lds = sizeof(T) * ((size_t)DENSE_SPLIT + hcap);
     auto fn = &k_ilu_ext_partials<TV, T, TY, IB>;
     if (lds > dense_wg_lds_limit()) throw FsError(FS_ERR_UNSUPPORTED, "ILU block halo too large");

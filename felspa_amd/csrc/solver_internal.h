@@ -99,6 +99,7 @@ struct IluDev {
   DevBuf xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32;
   int xl_ib = 1, xu_ib = 1;          // bytes per halo index of an external entry
   int xl_hmax = 0, xu_hmax = 0;      // largest halo of a block
+  int xl_phmax = 0, xu_phmax = 0;    // largest halo of a piece of a split block
   int64_t n_xlh = 0, n_xuh = 0;      // halo columns stored (padded)
   DevBuf xlsplit, xusplit, xlpieces, xupieces, dpart;  // pieces of blocks with many external chunks
   std::vector<int> xl_pc_ptr, xu_pc_ptr;               // per stage: range of pieces
