@@ -229,7 +229,7 @@ def main():
     # its stage launches); the class with the larger share of the solve is
     # reported, its unit being one SpMV launch or one ILU application.
     _, n_stages = mixed.ordering(0)
-    kinds = {0: "A-block SpMV (k_spmv4_dot, FP32)",
+    kinds = {0: "A-block SpMV (k_spmv_blk_dot, FP32)",
              2: "ILU(k) apply on A (k_ilu_stage_dense fwd+bwd, FP32)",
              1: "K SpMV (k_spmv4, FP64)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
     # (the last colour stage's forward and backward solves share one launch)
