@@ -221,7 +221,8 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4(int64_t nrows, int64_t row_off,
 // (i mod 8) * (n / 8) + i / 8 gives every XCD one contiguous range of the
 // work (rows, blocks), so the values it gathers from its neighbourhood stay
 // in its own L2; the tail beyond 8 (n / 8) keeps its numbers. (Used by the
-// ILU stages, 5 % faster at C2; the A-block SpMV measured slower with it.)
+// ILU stages, 5 % faster on the 2D 256x256 Q2 velocity system; the CSR4
+// A-block SpMV ran slower with it.)
 __device__ __forceinline__ int xcd_contiguous(int i, int n) {
   constexpr int NX = 8;
   const int per = n / NX;
