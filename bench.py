@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--ilu-k", type=int, default=None, help="override the configuration's ILU level on A")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64-vectors", action="store_true",
+                    help="skip the FP32-storage / FP64-vector leg (a third handle: host memory at 48^3)")
     ap.add_argument("--cpu-outer", type=int, default=2, help="outer iterations in the CPU sample")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start leg (C4 fields)")
     ap.add_argument("--learned-samples", type=int, default=4, help="held-out C4 fields in the learned leg")
@@ -184,7 +186,7 @@ def main():
     ms_mixed, st_mixed = timed(mixed, args.steps, args.warmup, "mixed")
     x_mixed = d_x.cpu().numpy().copy()
     out_extra = {}
-    if not args.no_plain:
+    if not args.no_plain and not args.no_fp64_vectors:
         # the same method with FP32-stored matrices and factors but the
         # preconditioner's vectors and arithmetic in FP64
         f64v = fa.StokesLinearSystem(S.blocks, precision=L.FS_PREC_MIXED_FP64_VECTORS, guess=L.FS_GUESS_ZERO,
