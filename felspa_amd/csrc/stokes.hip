@@ -216,6 +216,17 @@ struct fs_handle {
     cnt.bytes += 8.0 * len;
   }
 
+  // ||D^-1 v||_2 into sc[off] with D = Dv (the scaled system's residual
+  // norm), ||v||_2 when Dv is null; D^-1 v goes through dsc_t()
+  void wnorm(const double* v, int64_t len, const double* Dv, int off) {
+    if (!Dv) { norm(v, len, off); return; }
+    double* t = dsc_t();
+    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(len)), dim3(BLOCK), 0, st, len, v, Dv, t, 1);
+    cnt.launches++;
+    cnt.bytes += 24.0 * len;
+    norm(t, len, off);
+  }
+
   // y = K x (FP64) or y = b - K x
   void kmult64(const double* xv, double* y, const double* bv) {
     if (bv) spmv4_launch<double, double, double, double, double, 1>(K, n, 0, xv, y, bv, 0, st);
@@ -685,7 +696,10 @@ struct fs_handle {
   // InverseMatrixCG::vmult on A (linear_operators.implement.h:64-95): deal.II
   // SolverCG with ILU_A, ReductionControl(tol, reduce), from x (zero when
   // x_zero); written in the residual form r = b - A x, z = P r, p = z + beta p,
-  // whose iterates equal deal.II's g = -r, d = -h form.
+  // whose iterates equal deal.II's g = -r, d = -h form. With diagonal scaling
+  // the CG on A~ = D^-1 A D^-1 with ILU(A~) = D^-1 ILU(A) D^-1 is run in the
+  // unscaled variables (x = D^-1 x~, r = D r~): every inner product it takes
+  // is the same in both, and the stopping rule measures ||r~|| = ||D^-1 r||.
   int cg_A(const double* bv, double* xv, bool x_zero, double tol, double reduce, int max_steps, bool& ok) {
     double* base = (double*)pcg.p;
     double *rr = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
@@ -704,7 +718,7 @@ struct fs_handle {
     c.tol = tol;
     c.reduce = reduce;
     c.max_steps = max_steps;
-    norm(rr, nu, SC_TMP + 2);
+    wnorm(rr, nu, Dsc, SC_TMP + 2);
     Ctrl::State stt = c.check(0, read_scalar(SC_TMP + 2));
     int it = 0, cur = 0;
     if (stt == Ctrl::ITERATE) {
@@ -720,7 +734,7 @@ struct fs_handle {
       hipLaunchKernelGGL((k_pcg_update<double>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, xv, rr, 0);
       cnt.launches += 3;
       cnt.bytes += ablk_spmv_bytes(8.0, 8.0) + 8.0 * 5.0 * nu;
-      norm(rr, nu, SC_TMP + 2);
+      wnorm(rr, nu, Dsc, SC_TMP + 2);
       const double res = read_scalar(SC_TMP + 2);
       stt = c.check(it, res);
       if (stt != Ctrl::ITERATE) break;
@@ -739,6 +753,10 @@ struct fs_handle {
   // f - g; CG on S = B A^{-1} B^T preconditioned by InverseMatrixCG(M_p, ILU_S)
   // to rtol ||schur_rhs|| (ReductionControl with the solver's reduce); then
   // A u = f - B^T p from the current u. A^{-1} is CG with ILU_A to 1e-15.
+  // With diagonal scaling (pre_solve_scaling, stokes_ilu.cc:585-586) the
+  // method runs in the unscaled variables, as cg_A does: S~ = D_p^-1 S D_p^-1
+  // and M~^-1 = D_p M^-1 D_p leave every inner product of the CG unchanged,
+  // and its residual and right-hand-side norms are taken of D_p^-1 g.
   void scr(fs_stats& s) {
     const int64_t ldn = (n + 63) & ~int64_t(63), ldv = (nu + 63) & ~int64_t(63), ldp = (np + 63) & ~int64_t(63);
     if (!scr_ws.p) scr_ws.alloc(8 * (size_t)(ldn + 3 * ldv + 5 * ldp));
@@ -765,8 +783,13 @@ struct fs_handle {
     ainv(f, tmp, true);
     b_mult(tmp, srhs, full);
     axpby(np, -1.0, gp, 1.0, srhs);
+    const double* Dp = Dsc ? Dsc + nu : nullptr;
+    auto pnorm = [&](const double* v) {
+      wnorm(v, np, Dp, SC_TMP + 3);
+      return read_scalar(SC_TMP + 3);
+    };
     Ctrl c;
-    c.tol = o.rtol * std::sqrt(dot64(srhs, srhs, np));
+    c.tol = o.rtol * pnorm(srhs);
     c.reduce = o.reduce;
     c.max_steps = o.max_outer > 0 ? o.max_outer : (int)np;
     // deal.II SolverCG on S from x_p
@@ -777,7 +800,7 @@ struct fs_handle {
     } else {
       axpby(np, -1.0, srhs, 0.0, g);
     }
-    double res = std::sqrt(dot64(g, g, np));
+    double res = pnorm(g);
     Ctrl::State stt = c.check(0, res);
     int it = 0;
     double gh = 0;
@@ -793,7 +816,7 @@ struct fs_handle {
       if (!std::isfinite(alpha)) { stt = Ctrl::FAILURE; break; }
       axpby(np, alpha, d, 1.0, xp);
       axpby(np, alpha, hh, 1.0, g);
-      res = std::sqrt(dot64(g, g, np));
+      res = pnorm(g);
       stt = c.check(it, res);
       if (stt != Ctrl::ITERATE) break;
       inner_cg<double>(g, hh);
@@ -921,8 +944,6 @@ void fill_stats_err(fs_stats* s, int code) {
 int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   std::memset(&s, 0, sizeof(s));
   cnt = Counters{};
-  if (o.apply_diagonal_scaling && o.method == FS_METHOD_SCR)
-    throw FsError(FS_ERR_UNSUPPORTED, "diagonal scaling with SCR is not implemented on the GPU path");
   if (o.precision != FS_PREC_FP64 && (o.method == FS_METHOD_GMRES_REF || o.method == FS_METHOD_SCR))
     throw FsError(FS_ERR_UNSUPPORTED, "reference-mode GMRES and SCR run in FP64 only");
   hipEvent_t e0, e1, e2;
@@ -963,12 +984,16 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
   // tolerance is taken on the scaled rhs (stokes_ilu.cc:533)
   double bnorm_k = bnorm;
   Dsc = nullptr;
+  // (SCR keeps the unscaled vectors and weights its norms instead)
+  const bool scale_vectors = o.method != FS_METHOD_SCR;
   if (o.apply_diagonal_scaling && bnorm != 0.0) {
     compute_scaling();
-    scale_vec(dptr<double>(b), 1);
-    scale_vec(dptr<double>(x), 0);
-    norm(dptr<double>(b), n, SC_TMP);
-    bnorm_k = read_scalar(SC_TMP);
+    if (scale_vectors) {
+      scale_vec(dptr<double>(b), 1);
+      scale_vec(dptr<double>(x), 0);
+      norm(dptr<double>(b), n, SC_TMP);
+      bnorm_k = read_scalar(SC_TMP);
+    }
   }
   if (bnorm == 0.0) {
     hip_check(hipMemsetAsync(x.p, 0, 8 * n, st), "memset");
@@ -986,8 +1011,10 @@ int fs_handle::solve_device(const double* d_rhs, double* d_x, fs_stats& s) {
     throw FsError(FS_ERR_UNSUPPORTED, "solution method not implemented");
   }
   if (Dsc) {  // post_solve_scaling: x /= D; the rhs back to the caller's values
-    scale_vec(dptr<double>(x), 1);
-    scale_vec(dptr<double>(b), 0);
+    if (scale_vectors) {
+      scale_vec(dptr<double>(x), 1);
+      scale_vec(dptr<double>(b), 0);
+    }
     Dsc = nullptr;
   }
   hip_check(hipEventRecord(e2, st), "event record");

@@ -50,9 +50,26 @@ def test_scaled_fgmres_converges_to_the_unscaled_solution():
     assert e1 < max(10 * e0, 1e-6), (e0, e1)
 
 
-def test_scaling_with_scr_is_rejected():
-    S = system("C1", 4)
-    lin = fa.StokesLinearSystem(S.blocks, A_lof=0, method=L.FS_METHOD_SCR, apply_diagonal_scaling=True)
-    with pytest.raises(L.FsException) as e:
-        lin.solve(np.zeros(S.n), S.rhs)
-    assert e.value.code == L.FS_ERR_UNSUPPORTED
+@pytest.mark.parametrize("cfg,n,kA", [("C1", 8, 0), ("C2", 8, 1), ("C3", 3, 0)])
+def test_scaled_scr_matches_oracle(oracle, cfg, n, kA):
+    # solve_cg with pre_solve_scaling (src/pde/stokes_ilu.cc:584-586,673-678):
+    # the oracle scales the matrices; the GPU runs the Schur CG and its inner
+    # A^-1 CG in the unscaled variables and measures ||D^-1 r||
+    S = system(cfg, n)
+    lin = fa.StokesLinearSystem(S.blocks, A_lof=kA, method=L.FS_METHOD_SCR, ordering=L.FS_ORDER_REFERENCE,
+                                constraints=S.constrained, apply_diagonal_scaling=True)
+    x = np.zeros(S.n)
+    ctrl = fa.StokesSolverControl()
+    rhs0 = S.rhs.copy()
+    st = lin.solve(x, S.rhs, ctrl)
+    xo, sto = oracle.stokes_solve(S.blocks, S.rhs, kA=kA, kS=0, method=1, scaling=True)
+    assert sto["status"] == 0 and st["status"] == L.FS_OK
+    assert abs(st["outer_iters"] - sto["outer_iters"]) <= 1, (st["outer_iters"], sto["outer_iters"])
+    # the final scaled residual norms agree in magnitude (both just under the
+    # same tolerance on the same scaled right-hand side)
+    assert 0.1 < st["last_value"] / sto["last_value"] < 10, (st["last_value"], sto["last_value"])
+    assert rel(x[:S.n_u], xo[:S.n_u]) < 1e-6
+    dp, dpo = x[S.n_u:] - x[S.n_u:].mean(), xo[S.n_u:] - xo[S.n_u:].mean()
+    assert rel(dp, dpo) < 1e-5
+    # the caller's rhs is left as it was
+    assert np.array_equal(S.rhs, rhs0)
