@@ -53,6 +53,24 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;
 }
 
+// block_sum of two values at once (each summed in block_sum's order)
+__device__ __forceinline__ void block_sum2(double& a, double& b, double2* sh) {
+  a = group_reduce<WAVE>(a);
+  b = group_reduce<WAVE>(b);
+  const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  __syncthreads();
+  if (l == 0) sh[w] = make_double2(a, b);
+  __syncthreads();
+  double sa = 0, sb = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / WAVE); ++i) {
+      sa += sh[i].x;
+      sb += sh[i].y;
+    }
+  a = sa;
+  b = sb;
+}
+
 // ---------------------------------------------------------------------------
 // Grid barrier for cooperative launches: arrival counter + generation word,
 // with a wall-clock bounded wait so a stuck barrier ends the kernel with an
@@ -267,7 +285,8 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
                                                      const int32_t* __restrict__ rng, const void* __restrict__ xidx,
                                                      const TV* __restrict__ xv, const int32_t* __restrict__ cols,
                                                      const T* __restrict__ p, T* __restrict__ q,
-                                                     double* __restrict__ partial, int ccap) {
+                                                     double* __restrict__ partial, int ccap,
+                                                     const T* __restrict__ rdot, double* __restrict__ partial2) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
   __shared__ double sh[NT / WAVE];
   typedef TV V4 __attribute__((ext_vector_type(4)));
@@ -338,7 +357,7 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     }
   }
   __syncthreads();
-  double contrib = 0;
+  double contrib = 0, c2 = 0;
   if (tid < nr) {
     T s0 = 0, s1 = 0;
     for (int k = 0; k < cn; ++k) {
@@ -347,10 +366,21 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     }
     const T acc = s0 + s1;
     q[r0 + tid] = acc;
-    contrib = (double)acc * (double)p[r0 + tid];
+    const T pr = p[r0 + tid];
+    contrib = (double)acc * (double)pr;
+    if (rdot) c2 = (double)rdot[r0 + tid] * (double)pr;
   }
-  contrib = block_sum(contrib, sh);
-  if (tid == 0) partial[blk] = contrib;
+  if (rdot) {  // also partial2[blk] = sum rdot_i p_i over the block's rows
+    __shared__ double2 sh2[NT / WAVE];
+    block_sum2(contrib, c2, sh2);
+    if (tid == 0) {
+      partial[blk] = contrib;
+      partial2[blk] = c2;
+    }
+  } else {
+    contrib = block_sum(contrib, sh);
+    if (tid == 0) partial[blk] = contrib;
+  }
 }
 
 // CSR SpMV: G lanes per row, BLOCK/G rows per workgroup.
@@ -802,16 +832,21 @@ struct DenseTri {
   void* part;             // row partial sums of split blocks, S x rows each
 };
 
-// The update step of the inner PCG on A (alpha = rz / pq; x += alpha p, or
-// x = alpha p on the first step; r -= alpha q), carried out by the forward
-// stage kernels of the next ILU application on the rows they read r from:
-// every row belongs to exactly one stage, so the vectors are updated once.
+// The update step of the inner PCG on A in Chronopoulos-Gear form (one
+// reduction per step; s = A z, q = A p by recurrence):
+//   p = z + beta p, q = s + beta q   (p = z, q = s on the first step)
+//   x += alpha p (x = alpha p on the first step), r -= alpha q
+// carried out by the forward stage kernels of the next ILU application on the
+// rows they read r from: every row belongs to exactly one stage, so the
+// vectors are updated once.
 template <typename T>
 struct PcgFuse {
-  const double* pq;
-  const double* rz;
-  const T* p;
-  const T* q;
+  const double* alpha;  // device scalars of this step
+  const double* beta;
+  const T* z;  // z and s = A z of the previous step
+  const T* s;
+  T* p;
+  T* q;
   T* x;
   T* r;  // null: no update
   int first;
@@ -1064,14 +1099,16 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     const uint32_t rg = (uint32_t)F.rng[r0 + tid];
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
-    if (FWD && pf.r) {  // the PCG update of this row (k_pcg_update's arithmetic)
-      const double a = pf.pq[0] != 0.0 ? pf.rz[0] / pf.pq[0] : 0.0;
-      const T alpha = (T)a;
+    if (FWD && pf.r) {  // the PCG update of this row (k_cg1_update's arithmetic)
+      const T alpha = (T)pf.alpha[0], beta = (T)pf.beta[0];
       const int i = r0 + tid;
-      const T pi = pf.p[i];
+      const T zi = pf.z[i], si = pf.s[i];
+      const T pi = pf.first ? zi : zi + beta * pf.p[i];
+      const T qi = pf.first ? si : si + beta * pf.q[i];
+      pf.p[i] = pi;
+      pf.q[i] = qi;
       pf.x[i] = pf.first ? alpha * pi : pf.x[i] + alpha * pi;
-      T ri = pf.r[i];
-      ri -= alpha * pf.q[i];
+      const T ri = pf.r[i] - alpha * qi;
       pf.r[i] = ri;
       bi = ri;
     } else {
@@ -1614,6 +1651,57 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_direction(int64_t n, const double
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
     p[i] = first ? z[i] : z[i] + beta * p[i];
   if (blockIdx.x == 0 && threadIdx.x == 0) rz_new[0] = rzn;
+}
+
+// The inner PCG's update step in Chronopoulos-Gear form (the per-row
+// arithmetic the forward ILU stages apply when they carry it): p = z + beta p,
+// x += alpha p (p = z, x = alpha p when first); with r: also q = s + beta q,
+// r -= alpha q. Without r only x is formed (p is not stored).
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_cg1_update(int64_t n, const double* __restrict__ alpha_p,
+                                                      const double* __restrict__ beta_p, const T* __restrict__ z,
+                                                      const T* __restrict__ s, T* __restrict__ p, T* __restrict__ q,
+                                                      T* __restrict__ x, T* __restrict__ r, int first) {
+  const T alpha = (T)alpha_p[0], beta = (T)beta_p[0];
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const T zi = z[i];
+    const T pi = first ? zi : zi + beta * p[i];
+    x[i] = first ? alpha * pi : x[i] + alpha * pi;
+    if (r) {
+      const T si = s[i];
+      const T qi = first ? si : si + beta * q[i];
+      p[i] = pi;
+      q[i] = qi;
+      r[i] -= alpha * qi;
+    }
+  }
+}
+
+// The scalars of one Chronopoulos-Gear CG step (one workgroup): gamma =
+// sum(rz_part) = r.z, delta = sum(zs_part) = z.A z;
+//   first: beta = 0, alpha = gamma / delta
+//   else:  beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+// sc[0] = alpha, sc[1] = beta, sc[2] = gamma (read as alpha_old, gamma_old).
+static __global__ __launch_bounds__(1024) void k_cg1_scalars(int nr, const double* __restrict__ rz_part, int nd,
+                                                       const double* __restrict__ zs_part, double* __restrict__ sc,
+                                                       int first) {
+  __shared__ double sh[1024 / WAVE];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) a += rz_part[i];
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) b += zs_part[i];
+  const double gamma = block_sum(a, sh);
+  const double delta = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    const double a_old = sc[0], g_old = sc[2];
+    double beta = 0, den = delta;
+    if (!first && g_old != 0.0 && a_old != 0.0) {
+      beta = gamma / g_old;
+      den = delta - beta * gamma / a_old;
+    }
+    sc[0] = den != 0.0 ? gamma / den : 0.0;
+    sc[1] = beta;
+    sc[2] = gamma;
+  }
 }
 
 // r = (T) src

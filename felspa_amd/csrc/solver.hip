@@ -694,7 +694,7 @@ static void apply_dense(const IluDev& F, bool fuse, const TB* b, T* tmp, T* x, c
   const Schedule& sc = F.sched;
   const int ns = sc.n_stages();
   const int nf = fuse ? ns - 1 : ns;  // stages with a forward launch of their own
-  const PcgFuse<T> none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  const PcgFuse<T> none{};
   for (int s = 0; s < nf; ++s)
     launch_dense_g<TV, T, TB, T, DS_FWD>(F, sc.stage_ptr[s], sc.stage_ptr[s + 1], b, tmp, tmp, pf, st);
   if constexpr (sizeof(TV) == 4) {
@@ -710,7 +710,7 @@ static void apply_dense(const IluDev& F, bool fuse, const TB* b, T* tmp, T* x, c
 // as its own kernel ahead of the substitution kernels.
 template <typename TV, typename T, typename TB>
 void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt, const PcgFuse<T>* pf) const {
-  const PcgFuse<T> none{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  const PcgFuse<T> none{};
   const int ns = sched.n_stages();
   // the dense kernels keep a block's packed triangle (the fused last stage
   // both of them), its halo values and its chunk partial sums in LDS (room
@@ -734,8 +734,8 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt, con
   if (!std::is_same<TV, T>::value)
     throw FsError(FS_ERR_UNSUPPORTED, "ILU application with FP32 factors and FP64 vectors needs the dense kernels");
   if (pf && pf->r) {
-    hipLaunchKernelGGL((k_pcg_update<T>), dim3(256), dim3(BLOCK), 0, st, (int64_t)n, pf->pq, pf->rz, pf->p, pf->q, pf->x,
-                       pf->r, pf->first);
+    hipLaunchKernelGGL((k_cg1_update<T>), dim3(256), dim3(BLOCK), 0, st, (int64_t)n, pf->alpha, pf->beta, pf->z,
+                       pf->s, pf->p, pf->q, pf->x, pf->r, pf->first);
     cnt.launches++;
   }
   using TS = T;  // substitution kernels: factor values and vectors of one type

@@ -297,45 +297,39 @@ struct fs_handle {
 
   // BlockSchurPreconditionerBase::vmult (linear_operators.implement.h:200-209):
   // dst_u = ILU_A^{-1} src_u ; t = B dst_u - src_p ; dst_p = CG(M_p, ILU_S; t)
-  // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0.
-  // Scalars never leave the device: the update / direction kernels sum the
-  // producers' per-workgroup partials themselves (fixed order).
+  // A^{-1} src_u: o.inner_a_iters steps of ILU(k)-preconditioned CG from x = 0,
+  // in Chronopoulos-Gear form: one step is the ILU application z = M^-1 r
+  // (its forward stages first apply the previous step's update to the rows
+  // they read), the SpMV s = A z with z.s and r.z partials, and one
+  // workgroup forming alpha and beta from both sums.
+  // The same iterates as the textbook PCG (q = A p by recurrence), with one
+  // reduction instead of two and no direction kernel. Scalars never leave the
+  // device; sums are taken in a fixed order.
   template <typename TV, typename T>
   void inner_pcg_A(const double* src, T* x, int iters) {
     T* base = (T*)pcg.p;
     T *r = base, *z = base + ldu, *pv = base + 2 * ldu, *q = base + 3 * ldu, *y = base + 4 * ldu;
-    double* part_pq = dptr<double>(pcg_part);
-    double* part_rz = part_pq + spmv_dot_grid();
-    double* rz = S(SC_PCG + 1);  // two slots
-    const int gv = 512, gu = 256;
-    hipLaunchKernelGGL((k_copy_convert<double, T>), dim3(gv), dim3(BLOCK), 0, st, nu, src, r);
-    FA.apply<TV, T, T>(r, y, z, st, cnt);
-    hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
-    hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz, rz, z, pv, 1);
-    cnt.launches += 3;
-    int cur = 0;
+    T* sv = base + 5 * ldu;
+    double* part_zs = dptr<double>(pcg_part);
+    double* part_rz = part_zs + spmv_dot_grid();
+    double* sc = S(SC_PCG);  // alpha, beta, gamma
+    const int nd = spmv_dot_grid();
+    hipLaunchKernelGGL((k_copy_convert<double, T>), dim3(512), dim3(BLOCK), 0, st, nu, src, r);
+    cnt.launches++;
     for (int it = 0; it < iters; ++it) {
-      launch_spmv_dot<TV, T>(pv, q, part_pq);
-      hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, st, spmv_dot_grid(), part_pq, S(SC_PCG), 0);
+      // x, r (and p, q) of step it-1 inside the forward stages; z = M^-1 r
+      const PcgFuse<T> pf{sc, sc + 1, z, sv, pv, q, x, r, it == 1 ? 1 : 0};
+      FA.apply<TV, T, T>(r, y, z, st, cnt, it > 0 ? &pf : nullptr);
+      launch_spmv_dot<TV, T>(z, sv, part_zs, r, part_rz);
+      hipLaunchKernelGGL(k_cg1_scalars, dim3(1), dim3(1024), 0, st, nd, part_rz, nd, part_zs, sc, it == 0 ? 1 : 0);
       cnt.launches += 2;
-      cnt.bytes += ablk_spmv_bytes(sizeof(TV), sizeof(T)) + sizeof(T) * 5.0 * nu;
-      if (it + 1 == iters) {
-        hipLaunchKernelGGL((k_pcg_update<T>), dim3(gu), dim3(BLOCK), 0, st, nu, S(SC_PCG), rz + cur, pv, q, x, r,
-                           it == 0 ? 1 : 0);
-        cnt.launches++;
-        break;
-      }
-      // alpha = rz / pq, x += alpha p, r -= alpha q: done by the forward
-      // stages of the ILU application that reads r
-      const PcgFuse<T> pf{S(SC_PCG), rz + cur, pv, q, x, r, it == 0 ? 1 : 0};
-      FA.apply<TV, T, T>(r, y, z, st, cnt, &pf);
-      hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(gv), dim3(BLOCK), 0, st, nu, r, z, part_rz);
-      hipLaunchKernelGGL((k_pcg_direction<T>), dim3(gu), dim3(BLOCK), 0, st, nu, part_rz, gv, rz + cur,
-                         rz + (cur ^ 1), z, pv, 0);
-      cnt.launches += 2;
-      cnt.bytes += sizeof(T) * 5.0 * nu;
-      cur ^= 1;
+      cnt.bytes += ablk_spmv_bytes(sizeof(TV), sizeof(T), true) + sizeof(T) * (it > 0 ? 10.0 : 1.0) * nu;
     }
+    // the last step's x (p is not kept)
+    hipLaunchKernelGGL((k_cg1_update<T>), dim3(256), dim3(BLOCK), 0, st, nu, sc, sc + 1, z, (const T*)nullptr, pv,
+                       (T*)nullptr, x, (T*)nullptr, iters == 1 ? 1 : 0);
+    cnt.launches++;
+    cnt.bytes += sizeof(T) * 3.0 * nu;
   }
 
   int spmv_dot_grid() const {
@@ -345,16 +339,19 @@ struct fs_handle {
   // bytes of one A-block SpMV with s_v-byte values and s_x-byte vectors:
   // the blocked layout (values + indices, column lists, row ranges, p read
   // and q written) or the CSR4 one
-  double ablk_spmv_bytes(double sv, double sx) const {
+  double ablk_spmv_bytes(double sv, double sx, bool rdot = false) const {
+    const double r = rdot ? sx * nu : 0.0;  // the second dot's other vector
     if (Ablb.nb > 0)
-      return (double)Ablb.nnz * (sv + Ablb.ib) + 4.0 * (double)Ablb.n_cols + 4.0 * nu + 2.0 * nu * sx;
-    return Ablk.nnz * (sv + 4.0) + 8.0 * (nu + 1) + sx * 2.0 * nu;
+      return (double)Ablb.nnz * (sv + Ablb.ib) + 4.0 * (double)Ablb.n_cols + 4.0 * nu + 2.0 * nu * sx + r;
+    return Ablk.nnz * (sv + 4.0) + 8.0 * (nu + 1) + sx * 2.0 * nu + r;
   }
 
   // q = A p and the per-workgroup partials of p.q, on the padded A block;
   // TV selects the stored values (Ablk.v4_32 or Ablk.v4_64), T the vectors
   template <typename TV, typename T>
-  void launch_spmv_dot(const T* pv, T* q, double* part) {
+  // q = A p with per-workgroup partials of p.q in part and, with rdot, of
+  // rdot.p in part2 (spmv_dot_grid() of each)
+  void launch_spmv_dot(const T* pv, T* q, double* part, const T* rdot = nullptr, double* part2 = nullptr) {
     if (Ablb.nb > 0) {  // by lattice blocks: 128 threads for up to 64 rows, 512 above
       const bool d = std::is_same<TV, double>::value;
       const TV* v = d ? (const TV*)Ablb.v64.p : (const TV*)Ablb.v32.p;
@@ -367,7 +364,8 @@ struct fs_handle {
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
                 "hipFuncSetAttribute");                                                                        \
     hipLaunchKernelGGL(fn, dim3(Ablb.nb), dim3(NT), lds, st, Ablb.desc.as<i32x4>(), Ablb.cinfo.as<int2>(),     \
-                       Ablb.rng.as<int32_t>(), Ablb.idx.p, v, Ablb.cols.as<int32_t>(), pv, q, part, ccap);     \
+                       Ablb.rng.as<int32_t>(), Ablb.idx.p, v, Ablb.cols.as<int32_t>(), pv, q, part, ccap, rdot, \
+                       part2);                                                                                 \
   }
       if (Ablb.rmax <= 64) {
         if (Ablb.ib == 1) FS_SB(1, 128) else FS_SB(2, 128)
@@ -390,6 +388,10 @@ struct fs_handle {
       default: FS_SD(32)
     }
 #undef FS_SD
+    if (rdot) {
+      hipLaunchKernelGGL((k_dot_partial_t<T>), dim3(g), dim3(BLOCK), 0, st, nu, rdot, pv, part2);
+      cnt.launches++;
+    }
   }
 
   // TV: stored matrix / factor values; T: vectors and arithmetic
@@ -1111,8 +1113,9 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.cg.alloc(8 * 4 * ((h.np + 63) & ~int64_t(63)));
   h.vin.alloc(8 * h.ld);
   h.ldu = (h.nu + 63) & ~int64_t(63);
-  h.pcg.alloc(8 * 5 * h.ldu);
-  h.pcg_part.alloc(8 * (size_t)(h.spmv_dot_grid() + 1024));
+  h.pcg.alloc(8 * 6 * h.ldu);
+  // z.s (or p.q) partials, then r.z partials (as many, or 512)
+  h.pcg_part.alloc(8 * (size_t)(2 * h.spmv_dot_grid() + 1024));
   h.tickets.alloc(64);
   hip_check(hipMemset(h.tickets.p, 0, 64), "memset");
   h.partial.alloc(8 * RED_BLOCKS * MAXM);
@@ -1369,14 +1372,19 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
     h->cnt = Counters{};
     switch (kind) {
       case 0: {
-        if (f32)
-          h->launch_spmv_dot<float, float>((const float*)h->ytmp.p, (float*)base, dptr<double>(h->pcg_part));
-        else if (mixed)
-          h->launch_spmv_dot<float, double>(h->vin.as<double>(), (double*)base, dptr<double>(h->pcg_part));
-        else
-          h->launch_spmv_dot<double, double>(h->vin.as<double>(), (double*)base, dptr<double>(h->pcg_part));
+        // as in the inner PCG: q = A p with the p.q and r.p partials
+        double* pp = dptr<double>(h->pcg_part);
+        double* pp2 = pp + h->spmv_dot_grid();
+        if (f32) {
+          const float* pv = (const float*)h->ytmp.p;
+          h->launch_spmv_dot<float, float>(pv, (float*)base, pp, pv, pp2);
+        } else if (mixed) {
+          h->launch_spmv_dot<float, double>(h->vin.as<double>(), (double*)base, pp, h->vin.as<double>(), pp2);
+        } else {
+          h->launch_spmv_dot<double, double>(h->vin.as<double>(), (double*)base, pp, h->vin.as<double>(), pp2);
+        }
         const double sv = (mixed || f32) ? 4.0 : 8.0, sx = f32 ? 4.0 : 8.0;
-        h->cnt.bytes = h->ablk_spmv_bytes(sv, sx);
+        h->cnt.bytes = h->ablk_spmv_bytes(sv, sx, true);
         break;
       }
       case 1:

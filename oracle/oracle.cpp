@@ -497,6 +497,14 @@ void gmres_left(const Op& K, const Prec& P, int64_t n, double* x, const double* 
 
 // Inner PCG on A (fixed number of steps from zero), ILU_A preconditioned:
 // the A^{-1} of the block preconditioner in FS_METHOD_FGMRES.
+// The flexible method's A^{-1}: a fixed number of ILU(k)-preconditioned CG
+// steps from x = 0 (no reference counterpart; the reference applies ILU_A once,
+// linear_operators.implement.h:200-209), in the Chronopoulos-Gear form that
+// the GPU runs: z = M^-1 r, s = A z, gamma = r.z, delta = z.s;
+// beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old);
+// p = z + beta p, q = s + beta q, x += alpha p, r -= alpha q. These are the
+// textbook PCG iterates (q = A p by recurrence, p.Ap = delta - beta gamma /
+// alpha_old), which tests/test_cpu_oracle.py checks against a textbook PCG.
 struct InnerPcgA {
   const DMat* A;
   const or_ilu* P;
@@ -504,29 +512,38 @@ struct InnerPcgA {
   mutable long long applies = 0;
   void vmult(double* x, const double* b) const {
     const int64_t n = A->m;
-    std::vector<double> r(b, b + n), z(n), p(n), q(n);
-    ilu_vmult(*P, r.data(), z.data());
-    ++applies;
-    p = z;
-    double rz = dot(r.data(), z.data(), n);
+    std::vector<double> r(b, b + n), z(n), s(n), p(n), q(n);
     std::fill(x, x + n, 0.0);
+    double alpha = 0, beta = 0, gamma_old = 0;
     for (int it = 0; it < iters; ++it) {
-      ::vmult(*A, p.data(), q.data());
-      const double pq = dot(p.data(), q.data(), n);
-      const double alpha = pq != 0.0 ? rz / pq : 0.0;
+      if (it > 0) {
+        const bool first = it == 1;
 #pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < n; ++i) {
-        x[i] += alpha * p[i];
-        r[i] -= alpha * q[i];
+        for (int64_t i = 0; i < n; ++i) {
+          p[i] = first ? z[i] : z[i] + beta * p[i];
+          q[i] = first ? s[i] : s[i] + beta * q[i];
+          x[i] = first ? alpha * p[i] : x[i] + alpha * p[i];
+          r[i] -= alpha * q[i];
+        }
       }
-      if (it + 1 == iters) break;
       ilu_vmult(*P, r.data(), z.data());
       ++applies;
-      const double rzn = dot(r.data(), z.data(), n);
-      const double beta = rz != 0.0 ? rzn / rz : 0.0;
-      rz = rzn;
+      ::vmult(*A, z.data(), s.data());
+      const double gamma = dot(r.data(), z.data(), n), delta = dot(z.data(), s.data(), n);
+      double den = delta;
+      beta = 0;
+      if (it > 0 && gamma_old != 0.0 && alpha != 0.0) {
+        beta = gamma / gamma_old;
+        den = delta - beta * gamma / alpha;
+      }
+      alpha = den != 0.0 ? gamma / den : 0.0;
+      gamma_old = gamma;
+    }
+    const bool first = iters == 1;
 #pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    for (int64_t i = 0; i < n; ++i) {
+      const double pi = first ? z[i] : z[i] + beta * p[i];
+      x[i] = first ? alpha * pi : x[i] + alpha * pi;
     }
   }
 };
@@ -673,6 +690,13 @@ void or_scaling_coeffs(const or_csr* Ac, const or_csr* Btc, int64_t n_p, double*
   for (int64_t r = 0; r < nu; ++r) g2 += ghat[r] * ghat[r];
   ps *= std::sqrt(ps * g2) / (double)n_p;
   for (int64_t i = 0; i < n_p; ++i) s_p[i] = ps;
+}
+
+void or_inner_pcg_a(const or_csr* A, int kA, int iters, const double* b, double* x) {
+  const DMat M = from_csr(A);
+  or_ilu F;
+  ilu_initialize(F, M, kA, false, false);
+  InnerPcgA{&M, &F, iters}.vmult(x, b);
 }
 
 int or_block_schur_vmult(const or_csr* A, const or_csr* B, const or_csr* Mp, int kA, int kS,

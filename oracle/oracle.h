@@ -69,6 +69,10 @@ int or_stokes_solve(const or_csr* A, const or_csr* Bt, const or_csr* B, const or
 int or_block_schur_vmult(const or_csr* A, const or_csr* B, const or_csr* Mp, int kA, int kS,
                          double inner_tol, const double* src, double* dst, long long* inner_iters);
 
+/* The flexible method's A^{-1} (no reference counterpart): iters steps of
+ * ILU(kA)-preconditioned CG on A from x = 0. */
+void or_inner_pcg_a(const or_csr* A, int kA, int iters, const double* b, double* x);
+
 /* StokesScalingOperator::initialize (stokes_common.implement.h:13-53):
  * writes s_u[n_u] and s_p[n_p]. */
 void or_scaling_coeffs(const or_csr* A, const or_csr* Bt, int64_t n_p, double* s_u, double* s_p);

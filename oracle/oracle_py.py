@@ -61,6 +61,7 @@ def lib():
                                            P(C.c_longlong)]
         L.or_scaling_coeffs.argtypes = [P(OrCsr), P(OrCsr), C.c_int64, P(C.c_double),
                                         P(C.c_double)]
+        L.or_inner_pcg_a.argtypes = [P(OrCsr), C.c_int, C.c_int, P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -166,3 +167,12 @@ def scaling_coeffs(A, Bt, n_p):
     lib().or_scaling_coeffs(C.byref(ca.s), C.byref(cb.s), n_p, _ptr(su, C.c_double),
                             _ptr(sp_, C.c_double))
     return su, sp_
+
+
+def inner_pcg_a(A, b, kA=1, iters=30):
+    """The flexible method's A^{-1}: iters ILU(kA)-PCG steps from zero."""
+    ca = _Csr(A)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    lib().or_inner_pcg_a(C.byref(ca.s), int(kA), int(iters), _ptr(b, C.c_double), _ptr(x, C.c_double))
+    return x

@@ -143,7 +143,7 @@ def test_block_schur_closed_form(oracle):
     assert rel(dst[S.n_u:], p) < 1e-10
 
 
-@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("method", [0, 1, 2])
 def test_stokes_solve_matches_direct(oracle, method):
     S = system("C2", 8)
     truth = spla.spsolve(S.K().tocsc(), S.rhs)
@@ -151,6 +151,34 @@ def test_stokes_solve_matches_direct(oracle, method):
                                 method=method)
     assert rel(x, truth) < 1e-7
     assert st["outer_iters"] > 0
+
+
+@pytest.mark.parametrize("iters", [1, 2, 7, 25])
+def test_inner_pcg_is_textbook_pcg(oracle, iters):
+    """The flexible method's A^{-1} (Chronopoulos-Gear form) against the
+    textbook preconditioned CG (Saad, Alg. 9.1) with the same ILU(1), fixed
+    steps from zero: the same iterates in exact arithmetic."""
+    S = system("C2", 8)
+    A = S.blocks["A"]
+    M = oracle.SparseILU(A, 1)
+    b = np.random.default_rng(3).standard_normal(A.shape[0])
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = M.vmult(r)
+    p = z.copy()
+    rz = r @ z
+    for it in range(iters):
+        q = A @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        if it + 1 == iters:
+            break
+        z = M.vmult(r)
+        rzn = r @ z
+        p = z + (rzn / rz) * p
+        rz = rzn
+    assert rel(oracle.inner_pcg_a(A, b, kA=1, iters=iters), x) < 1e-10
 
 
 def test_scaling_leaves_solution_unchanged(oracle):
