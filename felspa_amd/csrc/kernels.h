@@ -1686,9 +1686,27 @@ static __global__ __launch_bounds__(1024) void k_cg1_scalars(int nr, const doubl
                                                        const double* __restrict__ zs_part, double* __restrict__ sc,
                                                        int first) {
   __shared__ double sh[1024 / WAVE];
+  // thread t sums entries t, t + 1024, ... of each array in that order, the
+  // loads of 16 such entries of both arrays in flight together (unconditional
+  // loads of a clamped index, nr, nd >= 1; entries past the end count 0)
+  constexpr int NT = 1024, U = 16;
   double a = 0, b = 0;
-  for (int i = threadIdx.x; i < nr; i += blockDim.x) a += rz_part[i];
-  for (int i = threadIdx.x; i < nd; i += blockDim.x) b += zs_part[i];
+  const int m = nr > nd ? nr : nd;
+  for (int i = threadIdx.x; i < m; i += U * NT) {
+    double v[U], w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = i + u * NT;
+      v[u] = rz_part[j < nr ? j : nr - 1];
+      w[u] = zs_part[j < nd ? j : nd - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = i + u * NT;
+      a += j < nr ? v[u] : 0.0;
+      b += j < nd ? w[u] : 0.0;
+    }
+  }
   const double gamma = block_sum(a, sh);
   const double delta = block_sum(b, sh);
   if (threadIdx.x == 0) {

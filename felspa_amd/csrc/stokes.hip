@@ -332,6 +332,16 @@ struct fs_handle {
     cnt.bytes += sizeof(T) * 3.0 * nu;
   }
 
+  // one ILU application as the inner PCG runs it, for kernel timing: z = M^-1 r
+  // with the update step (upd: r, p, q, x, s) in its forward stages
+  template <typename TV, typename T>
+  void time_ilu_step(T* upd, T* y, T* z) {
+    T *r = upd, *p = upd + ldu, *q = upd + 2 * ldu, *x = upd + 3 * ldu, *s = upd + 4 * ldu;
+    const PcgFuse<T> pf{S(SC_PCG), S(SC_PCG) + 1, z, s, p, q, x, r, 0};
+    FA.apply<TV, T, T>(r, y, z, st, cnt, &pf);
+    cnt.bytes += sizeof(T) * 10.0 * nu;  // z, s, p, q, x, r read; p, q, x, r written
+  }
+
   int spmv_dot_grid() const {
     return Ablb.nb > 0 ? Ablb.nb : (int)std::max<int64_t>(1, (nu * Ablk.G4 + BLOCK - 1) / BLOCK);
   }
@@ -1368,6 +1378,16 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
   hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, h->n, h->vin.as<double>(),
                      (float*)h->ytmp.p);
   void* base = h->pcg.p;
+  // kind 2: the update step's vectors r, p, q, x, s; r = the input vector,
+  // the rest and alpha = beta = 0, so every repetition sees the same data
+  DevBuf upd;
+  if (kind == 2) {
+    const size_t es = f32 ? 4 : 8;
+    upd.alloc(es * 5 * (size_t)h->ldu);
+    hip_check(hipMemsetAsync(upd.p, 0, es * 5 * (size_t)h->ldu, h->st), "memset");
+    hip_check(hipMemsetAsync(h->S(SC_PCG), 0, 16, h->st), "memset");
+    hip_check(hipMemcpyAsync(upd.p, f32 ? h->ytmp.p : h->vin.p, es * nu, hipMemcpyDeviceToDevice, h->st), "D2D");
+  }
   auto once = [&]() {
     h->cnt = Counters{};
     switch (kind) {
@@ -1391,14 +1411,9 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
         h->kmult64(h->vin.as<double>(), h->r.as<double>(), nullptr);
         break;
       case 2:
-        if (f32)
-          h->FA.apply<float, float, float>((const float*)h->ytmp.p, (float*)base, (float*)base + h->ldu, h->st, h->cnt);
-        else if (mixed)
-          h->FA.apply<float, double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st,
-                                             h->cnt);
-        else
-          h->FA.apply<double, double, double>(h->vin.as<double>(), (double*)base, (double*)base + h->ldu, h->st,
-                                              h->cnt);
+        if (f32) h->time_ilu_step<float, float>((float*)upd.p, (float*)base, (float*)base + h->ldu);
+        else if (mixed) h->time_ilu_step<float, double>((double*)upd.p, (double*)base, (double*)base + h->ldu);
+        else h->time_ilu_step<double, double>((double*)upd.p, (double*)base, (double*)base + h->ldu);
         break;
       case 3:
         if (f32) h->inner_cg<float>((const float*)h->ytmp.p + nu, (float*)base);

@@ -177,8 +177,10 @@ int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* 
  * class on the handle's own stream between HIP events, with the current
  * factorisation and solver-order data, and returns the mean duration of one
  * launch (ms) and its algorithmic HBM bytes. kind: 0 = A-block SpMV of the
- * inner PCG (q = A p with p.q), 1 = K SpMV (FP64 true residual), 2 = one
- * ILU(k) application on A (all stage launches), 3 = one inner CG on M_p. */
+ * inner PCG (q = A p with the p.q and r.p partials), 1 = K SpMV (FP64 true
+ * residual), 2 = one ILU(k) application on A as the inner PCG runs it (all
+ * stage launches, the forward stages carrying the PCG update step), 3 = one
+ * inner CG on M_p. */
 int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t reps, double* ms_per_launch,
                           double* bytes_per_launch);
 
