@@ -311,10 +311,13 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     vv[u] = __builtin_nontemporal_load(vsrc + cc);
   }
   int cs = 0, cn = 0;
+  T pr = 0, rr = 0;  // p and rdot of this thread's row, loaded early
   if (tid < nr) {
     const uint32_t rg = (uint32_t)rng[r0 + tid];
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
+    pr = p[r0 + tid];
+    if (rdot) rr = rdot[r0 + tid];
   }
   const i32x4* c4 = (const i32x4*)(cols + ci.x);
   for (int k = tid; 4 * k < ci.y; k += NT) {
@@ -366,9 +369,8 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     }
     const T acc = s0 + s1;
     q[r0 + tid] = acc;
-    const T pr = p[r0 + tid];
     contrib = (double)acc * (double)pr;
-    if (rdot) c2 = (double)rdot[r0 + tid] * (double)pr;
+    c2 = (double)rr * (double)pr;
   }
   if (rdot) {  // also partial2[blk] = sum rdot_i p_i over the block's rows
     __shared__ double2 sh2[NT / WAVE];
