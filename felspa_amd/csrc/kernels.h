@@ -1075,14 +1075,14 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
   const i32x4* usrc = MODE == DS_BOTH ? (const i32x4*)(uinv + mo) : nullptr;
   const IX* xsrc = (const IX*)F.xidx + d.z;
   const V4* vsrc = (const V4*)F.xv + d.z;
+  // unconditional loads of a clamped index (nv >= 1): a load under a branch
+  // into an element of iv makes the compiler wait for it before the next one
   i32x4 iv[RI], uv[RI];
 #pragma unroll
   for (int u = 0; u < RI; ++u) {
-    const int k = u * NT + tid;
-    if (k < nv) {
-      iv[u] = __builtin_nontemporal_load(isrc + k);
-      if (MODE == DS_BOTH) uv[u] = __builtin_nontemporal_load(usrc + k);
-    }
+    const int k = u * NT + tid < nv ? u * NT + tid : 0;
+    iv[u] = __builtin_nontemporal_load(isrc + k);
+    if (MODE == DS_BOTH) uv[u] = __builtin_nontemporal_load(usrc + k);
   }
   IX cx[RC];
   V4 vv[RC];
