@@ -886,10 +886,10 @@ __host__ __device__ constexpr int dense_rc() {
 // 4k..4k+3, 4(k + NT).., U groups of 4 in flight, from the finished values yin
 template <typename T, typename TY, int NT, int U>
 __device__ __forceinline__ void gather_halo(const int32_t* __restrict__ hcol, int nh, const TY* yin, T* hv,
-                                            int tid) {
+                                            int tid, int k0 = 0) {
   const i32x4* h4 = (const i32x4*)hcol;
   const int nu = (nh + 3) / 4;
-  for (int k = tid; k < nu; k += U * NT) {
+  for (int k = k0 + tid; k < nu; k += U * NT) {
     i32x4 c[U];
     TY v[U][4];
 #pragma unroll
@@ -1094,6 +1094,17 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       vv[u] = __builtin_nontemporal_load(vsrc + c);
     }
   }
+  // the first round of the halo's column indices in the same burst (from a
+  // valid address when the block has no halo: the values are then unused)
+  constexpr int UH = NT >= 512 ? 2 : 1;
+  const int nhu = (nh + 3) / 4;
+  const i32x4* h4 = nhu > 0 ? (const i32x4*)(F.hcol + hl.x) : F.desc;
+  i32x4 hc[UH];
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {
+    const int k = u * NT + tid;
+    hc[u] = __builtin_nontemporal_load(h4 + (k < nhu ? k : 0));
+  }
   // one row per thread (nr <= NT)
   int cs = 0, cn = 0;
   T bi = 0;
@@ -1104,13 +1115,16 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     if (FWD && pf.r) {  // the PCG update of this row (k_cg1_update's arithmetic)
       const T alpha = (T)pf.alpha[0], beta = (T)pf.beta[0];
       const int i = r0 + tid;
-      const T zi = pf.z[i], si = pf.s[i];
-      const T pi = pf.first ? zi : zi + beta * pf.p[i];
-      const T qi = pf.first ? si : si + beta * pf.q[i];
+      // every load before the first store (the pointers may alias, so the
+      // compiler keeps program order), unconditionally (values the first
+      // step does not use are discarded by the selects)
+      const T zi = pf.z[i], si = pf.s[i], po = pf.p[i], qo = pf.q[i], xo = pf.x[i], ro = pf.r[i];
+      const T pi = pf.first ? zi : zi + beta * po;
+      const T qi = pf.first ? si : si + beta * qo;
       pf.p[i] = pi;
       pf.q[i] = qi;
-      pf.x[i] = pf.first ? alpha * pi : pf.x[i] + alpha * pi;
-      const T ri = pf.r[i] - alpha * qi;
+      pf.x[i] = pf.first ? alpha * pi : xo + alpha * pi;
+      const T ri = ro - alpha * qi;
       pf.r[i] = ri;
       bi = ri;
     } else {
@@ -1121,6 +1135,17 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       for (int q = 0; q < sp.x; ++q) bi -= P[(size_t)q * nr];
     }
   }
+  // the first round's halo values (column 0 stands in past the end), in
+  // flight while the inverse goes to LDS
+  TY hy[UH][4];
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {
+    const bool in = u * NT + tid < nhu;
+    hy[u][0] = yin[in ? hc[u].x : 0];
+    hy[u][1] = yin[in ? hc[u].y : 0];
+    hy[u][2] = yin[in ? hc[u].z : 0];
+    hy[u][3] = yin[in ? hc[u].w : 0];
+  }
 #pragma unroll
   for (int u = 0; u < RI; ++u) {
     const int k = u * NT + tid;
@@ -1129,7 +1154,17 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
       if (MODE == DS_BOTH) ((i32x4*)Us)[k] = uv[u];
     }
   }
-  gather_halo<T, TY, NT, (NT >= 512 ? 2 : 1)>(F.hcol + hl.x, nh, yin, hv, tid);
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {
+    const int k = u * NT + tid;
+    if (k < nhu) {
+      hv[4 * k] = (T)hy[u][0];
+      hv[4 * k + 1] = (T)hy[u][1];
+      hv[4 * k + 2] = (T)hy[u][2];
+      hv[4 * k + 3] = (T)hy[u][3];
+    }
+  }
+  gather_halo<T, TY, NT, UH>(F.hcol + hl.x, nh, yin, hv, tid, UH * NT);
   // the rest of the inverse: two units per thread in flight
   for (int k = RI * NT + tid; k < nv; k += 2 * NT) {
     const bool two = k + NT < nv;
