@@ -310,17 +310,27 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     cx[u] = __builtin_nontemporal_load(isrc + cc);
     vv[u] = __builtin_nontemporal_load(vsrc + cc);
   }
-  int cs = 0, cn = 0;
-  T pr = 0, rr = 0;  // p and rdot of this thread's row, loaded early
-  if (tid < nr) {
-    const uint32_t rg = (uint32_t)rng[r0 + tid];
-    cs = (int)(rg & 0xffffu);
-    cn = (int)(rg >> 16);
-    pr = p[r0 + tid];
-    if (rdot) rr = rdot[r0 + tid];
-  }
+  // this thread's row range, p and rdot, and the first round of the column
+  // list, all in the first burst: unconditional loads of clamped indices
+  // (nr >= 1; the column list holds at least the block's own rows)
+  const int tr = tid < nr ? tid : nr - 1;
+  const uint32_t rg = (uint32_t)rng[r0 + tr];
+  const T pr = p[r0 + tr];
+  const T rr = (rdot ? rdot : p)[r0 + tr];
+  const int cs = tid < nr ? (int)(rg & 0xffffu) : 0;
+  const int cn = tid < nr ? (int)(rg >> 16) : 0;
   const i32x4* c4 = (const i32x4*)(cols + ci.x);
-  for (int k = tid; 4 * k < ci.y; k += NT) {
+  const i32x4 cf = __builtin_nontemporal_load(c4 + (4 * tid < ci.y ? tid : 0));
+  {
+    const T a0 = p[cf.x], a1 = p[cf.y], a2 = p[cf.z], a3 = p[cf.w];
+    if (4 * tid < ci.y) {
+      xs[4 * tid] = a0;
+      xs[4 * tid + 1] = a1;
+      xs[4 * tid + 2] = a2;
+      xs[4 * tid + 3] = a3;
+    }
+  }
+  for (int k = NT + tid; 4 * k < ci.y; k += NT) {
     const i32x4 c = __builtin_nontemporal_load(c4 + k);
     const T a0 = p[c.x], a1 = p[c.y], a2 = p[c.z], a3 = p[c.w];
     xs[4 * k] = a0;
@@ -370,7 +380,7 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     const T acc = s0 + s1;
     q[r0 + tid] = acc;
     contrib = (double)acc * (double)pr;
-    c2 = (double)rr * (double)pr;
+    c2 = rdot ? (double)rr * (double)pr : 0.0;
   }
   if (rdot) {  // also partial2[blk] = sum rdot_i p_i over the block's rows
     __shared__ double2 sh2[NT / WAVE];
