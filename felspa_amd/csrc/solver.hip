@@ -187,6 +187,8 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   // per-row offsets, and per-column counts of in-block entries
   for (int b = 0; b < sched.n_blocks(); ++b) {
     const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
+    // (the stage kernels read row r0 of every block unconditionally)
+    if (r1 <= r0) throw FsError(FS_ERR_BAD_INPUT, "ILU schedule with an empty block");
     for (int32_t r = r0; r < r1; ++r) {
       const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
       const int32_t* c = F.ci.data() + base;
@@ -835,6 +837,8 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
   cnmax = rmax = 0;
   for (int b = 0; b < nb; ++b) {
     const int32_t r0 = blk_ptr[b], r1 = blk_ptr[b + 1];
+    // (the SpMV reads row r0 and column-list entry 0 of every block unconditionally)
+    if (r1 <= r0) throw FsError(FS_ERR_BAD_INPUT, "SpMV blocking with an empty block");
     const int64_t qb = q;
     for (int32_t r = r0; r < r1; ++r) {
       const int64_t nc = (h.rp[r + 1] - h.rp[r] + 3) / 4;
@@ -865,6 +869,7 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
         hb.push_back(h.ci[e]);
       }
     std::sort(hb.begin(), hb.end());
+    if (hb.empty()) throw FsError(FS_ERR_BAD_INPUT, "SpMV blocking: a block without entries");
     if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "matrix block with too many columns");
     for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
     int64_t o = 4 * (int64_t)dsc[4 * b + 2];
