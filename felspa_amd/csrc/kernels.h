@@ -880,10 +880,11 @@ struct Idx4<2> {
   }
 };
 
-// Blocks with more than 2 DENSE_SPLIT chunks are summed in pieces of
+// Blocks with more than DENSE_SPLIT_ABOVE chunks are summed in pieces of
 // DENSE_SPLIT chunks by k_ilu_ext_partials, one workgroup per piece
-constexpr int DENSE_SPLIT = 768;
-constexpr int DENSE_SPLIT_NT = 256;
+constexpr int DENSE_SPLIT_ABOVE = 1536;
+constexpr int DENSE_SPLIT = 1536;
+constexpr int DENSE_SPLIT_NT = 512;
 
 // Chunks of external entries per thread loaded with the inverse in the
 // dense stage kernel; part_cap >= dense_rc * threads or >= every block's chunks

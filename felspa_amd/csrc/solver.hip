@@ -378,7 +378,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       for (int b = 0; b < nb; ++b) {
         const int nc = dsc[4 * b + 3];
         const int64_t a = 4 * (int64_t)dsc[4 * b + 2], z = a + 4 * (int64_t)nc;
-        if (nc > 2 * DENSE_SPLIT) {
+        if (nc > DENSE_SPLIT_ABOVE) {
           for (int c = 0; c < nc; c += DENSE_SPLIT) {
             int32_t off = 0, cnt = 0;
             halo(a + 4 * (int64_t)c, a + 4 * (int64_t)std::min(nc, c + DENSE_SPLIT), off, cnt);
@@ -432,7 +432,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       for (int st = 0; st < sched.n_stages(); ++st) {
         for (int b = sched.stage_ptr[st]; b < sched.stage_ptr[st + 1]; ++b) {
           const int nr = dsc[4 * b + 1], nc = dsc[4 * b + 3];
-          const int S = nc > 2 * DENSE_SPLIT ? (nc + DENSE_SPLIT - 1) / DENSE_SPLIT : 1;
+          const int S = nc > DENSE_SPLIT_ABOVE ? (nc + DENSE_SPLIT - 1) / DENSE_SPLIT : 1;
           sp[2 * b] = S;
           sp[2 * b + 1] = (int32_t)off;
           if (S > 1) {
