@@ -419,35 +419,6 @@ __global__ __launch_bounds__(BLOCK) void k_spmv(int64_t nrows, int64_t row_off,
   }
 }
 
-// Deterministic grid reduction: every workgroup stores its partial; the last
-// workgroup to finish (atomic ticket) sums all partials in index order and
-// writes the scalar, then resets the ticket for the next launch.
-__device__ __forceinline__ void grid_reduce_last(double s, double* __restrict__ partial,
-                                                 unsigned int* __restrict__ ticket, double* __restrict__ out,
-                                                 double* sh) {
-  s = block_sum(s, sh);
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-    partial[blockIdx.x] = s;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned int t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double t = 0;
-    // fixed order: thread i sums partials i, i+BLOCK, ..., then a fixed tree
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += BLOCK)
-      t += __hip_atomic_load(partial + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = block_sum(t, sh);
-    if (threadIdx.x == 0) {
-      out[0] = t;
-      __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // deterministic two-stage reductions
 template <typename T>

@@ -176,7 +176,7 @@ struct fs_handle {
   std::vector<int64_t> ablk_from_k;                       // A entry -> K position
   std::vector<double> hK, hMp, hA;                        // host value staging
   IluDev FA, FS;
-  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part, tickets;
+  DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
   int64_t ldu = 0;
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
@@ -1126,8 +1126,6 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.pcg.alloc(8 * 6 * h.ldu);
   // z.s (or p.q) partials, then r.z partials (as many, or 512)
   h.pcg_part.alloc(8 * (size_t)(2 * h.spmv_dot_grid() + 1024));
-  h.tickets.alloc(64);
-  hip_check(hipMemset(h.tickets.p, 0, 64), "memset");
   h.partial.alloc(8 * RED_BLOCKS * MAXM);
   h.sc.alloc(8 * SC_TOTAL);
   h.gbar.alloc(sizeof(GridBarrier));
