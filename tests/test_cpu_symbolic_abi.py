@@ -98,3 +98,31 @@ def test_solver_control_statistics(tmp_path):
     lines = open(str(tmp_path / "stats.csv")).read().splitlines()
     assert lines[0] == "gmres_error,gmres_timer,n_gmres_iter,"
     assert lines[1].startswith("1e-09,0.1,5")
+
+
+@pytest.mark.parametrize("cfg,n,k,nodes", [("C1", 8, 0, None), ("C2", 16, 1, 5), ("C3", 4, 1, 3)])
+def test_block_color_schedule_is_valid(cfg, n, k, nodes):
+    """The schedule the GPU ILU runs (fs_block_color_ordering): a permutation,
+    blocks of at most 256 rows, and no ILU(k) factor entry joining two blocks
+    of one colour stage in the permuted matrix (the blocks of a stage are
+    solved at once)."""
+    S = system(cfg, n)
+    A = S.blocks["A"]
+    hint = S.block_hint(nodes) if nodes else None
+    perm, blk, stg = fa.block_color_ordering(A, k, 0 if nodes else 64, hint)
+    m = A.shape[0]
+    assert np.array_equal(np.sort(perm), np.arange(m))
+    sizes = np.diff(blk)
+    assert blk[0] == 0 and blk[-1] == m and sizes.min() >= 1 and sizes.max() <= 256
+    assert stg[0] == 0 and stg[-1] == len(blk) - 1 and np.diff(stg).min() >= 1
+    B = A[perm][:, perm].tocsr()
+    B.sort_indices()
+    F = fa.ilu_symbolic(B, k).tocoo()
+    block_of = np.repeat(np.arange(len(blk) - 1), sizes)
+    stage_of = np.repeat(np.arange(len(stg) - 1), np.diff(stg))
+    bi, bj = block_of[F.row], block_of[F.col]
+    assert not np.any((bi != bj) & (stage_of[bi] == stage_of[bj]))
+    # every row's entries in other blocks lie in earlier stages (lower part)
+    # or later stages (upper part): the forward sweep needs only finished stages
+    lower = F.col < F.row
+    assert np.all(stage_of[bj[lower & (bi != bj)]] < stage_of[bi[lower & (bi != bj)]])
