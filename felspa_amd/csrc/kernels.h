@@ -1269,61 +1269,75 @@ static __global__ __launch_bounds__(BLOCK) void k_ilu_block_inverse(IluView<doub
 // ---------------------------------------------------------------------------
 // Numeric ILU (include/felspa/linear_algebra/src/ilu.implement.h:438-541):
 // each workgroup of NT threads factorises the rows of one block in order.
-// The row is held in LDS (columns, working values, and a hash map column ->
-// position); the source matrix row (columns < col_limit) is scattered in,
-// then for each L column k in ascending order m = w_k / u_kk and
-// w_j -= m u_kj for the columns j > k of row k that row i stores. Rows k < i
-// are final, so every row k's U span and pivot are read for all L columns at
-// once, and row k+1's U part loads while row k's is applied (the barrier
-// between L columns waits for LDS only). FP64 throughout; err[0] is set on a
-// zero or non-finite pivot.
-__host__ __device__ inline int ilu_numeric_hash_size(int max_len) {
-  int h = 64;
-  while (h < 4 * max_len) h *= 2;  // load factor <= 1/4: lookups of dropped
-                                   // fill entries stay short
-  return h;
-}
-// working row (plus one slot that absorbs the updates of columns the row
-// does not store), per-L-column U spans and pivots, columns, hash map
-__host__ __device__ inline size_t ilu_numeric_lds_bytes(int max_len) {
-  return (size_t)(max_len + 2) * 8 + (size_t)max_len * (8 + 8 + 4 + 4) + (size_t)ilu_numeric_hash_size(max_len) * 8;
+// The row is held in LDS: its columns, its working values and a hash map
+// column -> position. The source matrix row (columns < col_limit) is
+// scattered in; then for each L column k in ascending order m = w_k / u_kk
+// and w_j -= m u_kj for the columns j > k of row k that row i stores. Rows
+// k < i are final. For every L column the start and length of row k's U part
+// and the pivot u_kk sit in an LDS ring of two windows of `win` columns,
+// refilled one window ahead; row k's U part loads three L columns ahead of
+// its use (the barrier between L columns waits for LDS only). FP64
+// throughout; err[0] is set on a zero or non-finite pivot.
+//
+// Hash slots are 32 bits: a 16-bit tag of the column's hash above the
+// position in the row (rows of at most 65534 entries); a slot whose tag
+// matches is confirmed against the column list. Empty slots are all ones.
+constexpr uint32_t NUM_EMPTY = 0xFFFFFFFFu;
+constexpr int NUM_MAX_LEN = 65534;
+
+// LDS plan for rows of up to max_len entries of which up to max_nl lie left
+// of the diagonal: the L-column window (a power of two, 64..512), the hash
+// size (a power of two, load factor <= 1/4 when it fits the limit, never
+// above 1/2) and the bytes; 0 bytes when no plan fits.
+__host__ __device__ inline size_t ilu_numeric_plan(int max_len, int max_nl, size_t limit, int& hs, int& win) {
+  win = 64;
+  while (win < max_nl && win < 512) win *= 2;
+  const size_t fixed = (size_t)(max_len + 1) * 8 + (size_t)2 * win * (8 + 8 + 4) + (size_t)max_len * 4;
+  hs = 64;
+  while (hs < 4 * max_len) hs *= 2;
+  while (fixed + (size_t)4 * hs > limit && hs / 2 >= 2 * max_len) hs /= 2;
+  const size_t bytes = fixed + (size_t)4 * hs;
+  return (bytes <= limit && max_len <= NUM_MAX_LEN) ? bytes : 0;
 }
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
-// multiplicative (Fibonacci) hashing: the top bits of c * 2^32 / phi
-__device__ __forceinline__ uint32_t ilu_hash(int32_t c, uint32_t mask) {
-  return ((uint32_t)c * 2654435769u) >> (32 - __builtin_popcount(mask));
-}
-// hash map slots {column, position}, linear probing; column -1 = empty
-__device__ __forceinline__ int ilu_probe(const int2* tab, uint32_t mask, int32_t c, uint32_t h, int2 e) {
-  while (e.x != c && e.x >= 0) {
+// multiplicative (Fibonacci) hashing: the top lg bits of c * 2^32 / phi pick
+// the slot, the low 16 bits are the tag
+__device__ __forceinline__ uint32_t num_hash(int32_t c) { return (uint32_t)c * 2654435769u; }
+// linear probing from slot h, first slot already read into e
+__device__ __forceinline__ int num_probe(const uint32_t* tab, const int32_t* lc, uint32_t mask, int32_t c,
+                                         uint32_t tag, uint32_t h, uint32_t e) {
+  while (e != NUM_EMPTY) {
+    if ((e >> 16) == tag && lc[e & 0xFFFFu] == c) return (int)(e & 0xFFFFu);
     h = (h + 1) & mask;
     e = tab[h];
   }
-  return e.x == c ? e.y : -1;
+  return -1;
 }
-__device__ __forceinline__ int ilu_find(const int2* tab, uint32_t mask, int32_t c) {
-  const uint32_t h = ilu_hash(c, mask);
-  return ilu_probe(tab, mask, c, h, tab[h]);
+__device__ __forceinline__ int num_find(const uint32_t* tab, const int32_t* lc, uint32_t mask, int lg, int32_t c) {
+  const uint32_t hh = num_hash(c);
+  const uint32_t h = hh >> (32 - lg);
+  return num_probe(tab, lc, mask, c, hh & 0xFFFFu, h, tab[h]);
 }
 template <int NT, int M>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int blk0, int blk1,
                                                           const int64_t* __restrict__ srp,
                                                           const int32_t* __restrict__ sci,
                                                           const double* __restrict__ sv, int64_t col_limit,
-                                                          int max_len, int* __restrict__ err) {
+                                                          int max_len, int hs, int win, int* __restrict__ err) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  const int hs = ilu_numeric_hash_size(max_len);
   const uint32_t mask = (uint32_t)hs - 1;
-  double* w = (double*)smem;                 // working row; w[max_len]: dump slot
-  int64_t* qs = (int64_t*)(w + max_len + 2); // per L column k: first U entry of row k
-  double* uk = (double*)(qs + max_len);      // per L column k: pivot u_kk
-  int2* tab = (int2*)(uk + max_len);         // hash map column -> position
-  int32_t* lc = (int32_t*)(tab + hs);        // columns of the row
-  int32_t* qn = lc + max_len;                // per L column k: U entries of row k
+  const int lg = __builtin_ctz((uint32_t)hs);
+  const int ring = 2 * win, rmask = ring - 1;
+  double* w = (double*)smem;                  // working row; w[max_len]: dump slot
+  int64_t* mq = (int64_t*)(w + max_len + 1);  // ring: first U entry of row k
+  double* mu = (double*)(mq + ring);          // ring: pivot u_kk
+  int32_t* mn = (int32_t*)(mu + ring);        // ring: U entries of row k
+  int32_t* lc = mn + ring;                    // columns of the row
+  uint32_t* tab = (uint32_t*)(lc + max_len);  // hash map
   const int tid = threadIdx.x;
   const int blk = blk0 + blockIdx.x;
   if (blk >= blk1) return;
@@ -1332,33 +1346,43 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     const int64_t base = F.rp[r];
     const int len = (int)(F.rp[r + 1] - base);
     const int nl = F.o_diag[r];
-    for (int h = tid; h < hs; h += NT) tab[h] = make_int2(-1, 0);
+    for (int h = tid; h < hs; h += NT) tab[h] = NUM_EMPTY;
     lds_barrier();
     for (int t = tid; t < len; t += NT) {
       const int32_t c = F.ci[base + t];
       lc[t] = c;
       w[t] = 0.0;
-      for (uint32_t h = ilu_hash(c, mask);; h = (h + 1) & mask)
-        if (atomicCAS(&tab[h].x, -1, c) == -1) {
-          tab[h].y = t;
-          break;
-        }
-      if (t < nl) {
-        const int64_t kb = F.rp[c];
-        const int kd = F.o_diag[c];
-        qs[t] = kb + kd + 1;
-        qn[t] = (int)(F.rp[c + 1] - kb - kd - 1);
-        uk[t] = F.v[kb + kd];
-      }
+      const uint32_t hh = num_hash(c);
+      const uint32_t ent = ((hh & 0xFFFFu) << 16) | (uint32_t)t;
+      for (uint32_t h = hh >> (32 - lg);; h = (h + 1) & mask)
+        if (atomicCAS(&tab[h], NUM_EMPTY, ent) == NUM_EMPTY) break;
     }
     lds_barrier();
+    // the U part and pivot of every L column of window wi
+    auto load_window = [&](int wi) {
+      for (int j = tid; j < win; j += NT) {
+        const int t = wi * win + j;
+        if (t < nl) {
+          const int32_t k = lc[t];
+          const int64_t kb = F.rp[k];
+          const int kd = F.o_diag[k];
+          const int s = t & rmask;
+          mq[s] = kb + kd + 1;
+          mn[s] = (int)(F.rp[k + 1] - kb - kd - 1);
+          mu[s] = F.v[kb + kd];
+        }
+      }
+    };
+    load_window(0);
+    load_window(1);
     for (int64_t p = srp[r] + tid; p < srp[r + 1]; p += NT) {
       const int32_t c = sci[p];
       if (c >= col_limit) continue;
-      const int pos = ilu_find(tab, mask, c);
+      const int pos = num_find(tab, lc, mask, lg, c);
       if (pos >= 0) w[pos] = sv[p];
     }
     lds_barrier();
+    int loaded = 1;  // last window in the ring
     // three register sets in rotation: the U part of L column t + 3 loads
     // while columns t + 1 and t + 2 are applied
     int32_t ja[M], jb[M], jc[M];
@@ -1366,12 +1390,11 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     // (every load is issued, from entry 0 when out of range, so the number
     // of loads in flight is the same on every path and the waits before a
     // set's use leave the two younger sets in flight)
-    // (nothing looks at a loaded value until it is used)
     auto fetch = [&](int t, int32_t* j, double* v) {
       const bool ok = t < nl;
-      const int tt = ok ? t : 0;
-      const int64_t q = qs[tt];
-      const int n = ok ? qn[tt] : 0;
+      const int s = (ok ? t : 0) & rmask;
+      const int64_t q = mq[s];
+      const int n = ok ? mn[s] : 0;
 #pragma unroll
       for (int m = 0; m < M; ++m) {
         const int idx = tid + m * NT;
@@ -1380,35 +1403,59 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
         v[m] = F.v[a];
       }
     };
-    auto apply = [&](int t, const int32_t* j, const double* v) {
-      // w[t] is final: later L columns only update positions > t
-      const double mult = w[t] / uk[t];
-      const int n = qn[t];
-      // first probes of all M lookups together, then the rare collisions;
-      // entries out of range or not stored update the dump slot, so the
-      // reads and writes of w issue back to back
-      uint32_t h[M];
-      int2 e[M];
+    // w_j -= mult u_kj for M entries of row k's U part, first probes of all
+    // M lookups together, then the rare collisions; entries out of range or
+    // not stored update the dump slot, so the reads and writes of w issue
+    // back to back
+    auto update = [&](double mult, int n, int off, const int32_t* j, const double* v) {
+      uint32_t h[M], e[M];
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        h[m] = ilu_hash(j[m], mask);
+        h[m] = num_hash(j[m]) >> (32 - lg);
         e[m] = tab[h[m]];
       }
       int pos[M];
 #pragma unroll
       for (int m = 0; m < M; ++m) {
-        const int p = ilu_probe(tab, mask, j[m], h[m], e[m]);
-        pos[m] = tid + m * NT < n && p >= 0 ? p : max_len;
+        const int p = num_probe(tab, lc, mask, j[m], num_hash(j[m]) & 0xFFFFu, h[m], e[m]);
+        pos[m] = off + tid + m * NT < n && p >= 0 ? p : max_len;
       }
       double o[M];
 #pragma unroll
       for (int m = 0; m < M; ++m) o[m] = w[pos[m]];
 #pragma unroll
       for (int m = 0; m < M; ++m) w[pos[m]] = o[m] - mult * v[m];
-      const int64_t q = qs[t];
-      for (int idx = M * NT + tid; idx < n; idx += NT) {  // U parts longer than M NT
-        const int p = ilu_find(tab, mask, F.ci[q + idx]);
-        if (p >= 0) w[p] -= mult * F.v[q + idx];
+    };
+    auto apply = [&](int t, const int32_t* j, const double* v) {
+      // w[t] is final: later L columns only update positions > t
+      const int s = t & rmask;
+      const double mult = w[t] / mu[s];
+      const int n = mn[s];
+      if (tid == 0) F.v[base + t] = mult;  // l_it is final (no row reads row r's L part)
+      update(mult, n, 0, j, v);
+      if (n > M * NT) {  // U parts longer than M NT: further rounds of M entries
+        const int64_t q = mq[s];
+        for (int off = M * NT; off < n; off += M * NT) {
+          int32_t jt[M];
+          double vt[M];
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            const int idx = off + tid + m * NT;
+            const int64_t a = idx < n ? q + idx : 0;
+            jt[m] = F.ci[a];
+            vt[m] = F.v[a];
+          }
+          update(mult, n, off, jt, vt);
+        }
+      }
+    };
+    // the window after the current one, once every column still to be
+    // applied or fetched lies in the current window or later (win >= 64, so
+    // the fetches up to t + 5 never reach the window being loaded)
+    auto refill = [&](int t) {
+      if (t >= loaded * win && (loaded + 1) * win < nl) {
+        ++loaded;
+        load_window(loaded);
       }
     };
     fetch(0, ja, va);
@@ -1417,6 +1464,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     int t = 0;
     for (; t + 2 < nl; t += 3) {  // (no exit inside the body: every path
                                   // into the loop has the same loads in flight)
+      refill(t);
       apply(t, ja, va);
       fetch(t + 3, ja, va);
       lds_barrier();
@@ -1435,7 +1483,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
       apply(t + 1, jb, vb);
       lds_barrier();
     }
-    for (int t = tid; t < len; t += NT) F.v[base + t] = t < nl ? w[t] / uk[t] : w[t];
+    for (int t2 = nl + tid; t2 < len; t2 += NT) F.v[base + t2] = w[t2];
     if (tid == 0) {
       const double d = w[nl];
       if (!(d != 0.0) || !isfinite(d)) atomicExch(err, 1);

@@ -117,6 +117,27 @@ Schedule schedule_block_color(const HostPattern& A, int k, int block_rows, const
       adj[b] = nbs;
     }
   }
+  // a nonsymmetric pattern reaches a block from blocks it does not reach
+  // itself: join every list with its transpose, so two blocks within k+1
+  // hops of each other in either direction never share a colour
+  {
+    std::vector<int64_t> tc(nb + 1, 0);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int32_t a : adj[b]) tc[a + 1]++;
+    for (int64_t b = 0; b < nb; ++b) tc[b + 1] += tc[b];
+    std::vector<int32_t> tl(tc[nb]);
+    std::vector<int64_t> fill(tc.begin(), tc.end() - 1);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int32_t a : adj[b]) tl[fill[a]++] = (int32_t)b;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t b = 0; b < nb; ++b) {
+      auto& l = adj[b];
+      const size_t own = l.size();
+      l.insert(l.end(), tl.begin() + tc[b], tl.begin() + tc[b + 1]);
+      std::inplace_merge(l.begin(), l.begin() + own, l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+    }
+  }
   // greedy colouring in block order
   std::vector<int32_t> color(nb, -1);
   int ncolor = 0;
@@ -212,8 +233,9 @@ void row_with_diag(const HostPattern& A, int64_t i, std::vector<int32_t>& out) {
 
 }  // namespace
 
-HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched) {
+HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched, bool* stage_conflict) {
   const int64_t n = A.n;
+  if (stage_conflict) *stage_conflict = false;
   RowList rows(n);
   if (k <= 1) {
 #pragma omp parallel
@@ -235,13 +257,33 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched) {
       }
     }
   } else {
-    // levels of every stored entry (fits in 8 bits: levels never exceed k <= 255)
+    // Row i is stored as [L part by column][diagonal][U part by (level,
+    // column)] with the levels alongside, and ucnt[i k + l] = number of its U
+    // entries of level <= l (l < k). A left neighbour kk of level lk can only
+    // insert columns of row kk whose level is <= k - lk - 1 (any other gives a
+    // level above k), so only that prefix of row kk's U part is scanned. The
+    // insertions of one kk are independent of the order in which its columns
+    // are visited, so the pattern and levels equal those of a full scan.
     std::vector<std::vector<uint8_t>> lev(n);
     std::vector<int32_t> dpos(n);
     const uint8_t INV = 255;
     if (k > 254) k = 254;
-    auto do_row = [&](int64_t i, std::vector<uint8_t>& mark, std::vector<int32_t>& cols,
-                      std::vector<int32_t>& ri) {
+    std::vector<uint32_t> ucnt((size_t)n * (size_t)k);
+    // stage-parallel rows may only read rows of earlier stages or of their
+    // own block; a schedule that breaks this is reported, never read through
+    std::vector<int32_t> blk_of, stage_of;
+    const bool staged = sched && sched->n == n;
+    if (staged) {
+      blk_of.resize(n);
+      stage_of.resize(sched->n_blocks());
+      for (int b = 0; b < sched->n_blocks(); ++b)
+        for (int32_t r = sched->blk_ptr[b]; r < sched->blk_ptr[b + 1]; ++r) blk_of[r] = b;
+      for (int st = 0; st < sched->n_stages(); ++st)
+        for (int b = sched->stage_ptr[st]; b < sched->stage_ptr[st + 1]; ++b) stage_of[b] = st;
+    }
+    int conflict = 0;
+    auto do_row = [&](int64_t i, std::vector<uint8_t>& mark, std::vector<int32_t>& cols, std::vector<int32_t>& ri,
+                      std::vector<uint64_t>& up, int& bad) {
       row_with_diag(A, i, ri);
       cols.clear();
       std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>> heap;
@@ -255,9 +297,15 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched) {
         heap.pop();
         const int lk = mark[kk];
         if (lk >= k) continue;
+        if (staged && blk_of[kk] != blk_of[i] && stage_of[blk_of[kk]] >= stage_of[blk_of[i]]) {
+          bad = 1;  // row kk may still be in the making on another thread
+          continue;
+        }
         const auto& rk = rows[kk];
         const auto& lv = lev[kk];
-        for (size_t p = (size_t)dpos[kk] + 1; p < rk.size(); ++p) {
+        const size_t p0 = (size_t)dpos[kk] + 1;
+        const size_t p1 = p0 + ucnt[(size_t)kk * k + (size_t)(k - lk - 1)];
+        for (size_t p = p0; p < p1; ++p) {
           const int32_t j = rk[p];
           if (mark[j] != INV) continue;  // an existing entry keeps its first level
           const int L = lv[p] + lk + 1;
@@ -268,33 +316,62 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched) {
           }
         }
       }
-      std::sort(cols.begin(), cols.end());
-      std::vector<uint8_t> lvl(cols.size());
-      for (size_t p = 0; p < cols.size(); ++p) {
-        lvl[p] = mark[cols[p]];
-        mark[cols[p]] = INV;
-        if (cols[p] == i) dpos[i] = (int32_t)p;
+      // L part by column, the diagonal, U part by (level, column)
+      up.clear();
+      std::vector<int32_t> row;
+      row.reserve(cols.size());
+      for (int32_t c : cols) {
+        if (c < i) row.push_back(c);
+        else if (c > i) up.push_back(((uint64_t)mark[c] << 32) | (uint32_t)c);
       }
-      rows[i] = cols;
+      std::sort(row.begin(), row.end());
+      std::sort(up.begin(), up.end());
+      const size_t nl = row.size();
+      row.push_back((int32_t)i);
+      for (uint64_t e : up) row.push_back((int32_t)(uint32_t)e);
+      std::vector<uint8_t> lvl(row.size());
+      uint32_t* uc = ucnt.data() + (size_t)i * k;
+      for (int l = 0; l < k; ++l) uc[l] = 0;
+      for (size_t p = 0; p < row.size(); ++p) {
+        const int32_t c = row[p];
+        lvl[p] = mark[c];
+        mark[c] = INV;
+        if (c > i)
+          for (int l = lvl[p]; l < k; ++l) uc[l]++;
+      }
+      dpos[i] = (int32_t)nl;
+      rows[i].swap(row);
       lev[i].swap(lvl);
     };
-    if (sched && sched->n == n) {
-      for (int s = 0; s < sched->n_stages(); ++s) {
+    if (staged) {
+      for (int s = 0; s < sched->n_stages() && !conflict; ++s) {
         const int b0 = sched->stage_ptr[s], b1 = sched->stage_ptr[s + 1];
-#pragma omp parallel
+#pragma omp parallel reduction(| : conflict)
         {
           std::vector<uint8_t> mark(n, INV);
           std::vector<int32_t> cols, ri;
+          std::vector<uint64_t> up;
+          int bad = 0;
 #pragma omp for schedule(dynamic, 1)
           for (int b = b0; b < b1; ++b)
-            for (int32_t i = sched->blk_ptr[b]; i < sched->blk_ptr[b + 1]; ++i) do_row(i, mark, cols, ri);
+            for (int32_t i = sched->blk_ptr[b]; i < sched->blk_ptr[b + 1]; ++i) do_row(i, mark, cols, ri, up, bad);
+          conflict |= bad;
         }
       }
     } else {
       std::vector<uint8_t> mark(n, INV);
       std::vector<int32_t> cols, ri;
-      for (int64_t i = 0; i < n; ++i) do_row(i, mark, cols, ri);
+      std::vector<uint64_t> up;
+      int bad = 0;
+      for (int64_t i = 0; i < n; ++i) do_row(i, mark, cols, ri, up, bad);
     }
+    if (conflict) {
+      if (stage_conflict) *stage_conflict = true;
+      return HostPattern{};
+    }
+    // U parts back to column order
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) std::sort(rows[i].begin() + dpos[i] + 1, rows[i].end());
   }
   HostPattern F;
   F.n = n;

@@ -50,8 +50,11 @@ HostCsr permute_square(const HostCsr& A, const std::vector<int32_t>& perm,
 // ilu.implement.h:90-435): k = 0 -> the matrix pattern; k = 1 -> row i gets
 // A(i,:) and, for each k < i in A(i,:), the columns of A(k,:) right of k;
 // k >= 2 -> generic level recursion, a fill keeping the level it was first
-// inserted with. `sched` (may be null) lets independent blocks run in parallel.
-HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched);
+// inserted with. `sched` (may be null) lets independent blocks run in parallel;
+// when a row of the schedule would need a row of another block of its own
+// stage (the schedule does not separate the factor's blocks), the result is
+// empty and *stage_conflict (if given) is set.
+HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched, bool* stage_conflict = nullptr);
 
 // true when no entry of F joins two different blocks of the same stage
 bool schedule_is_valid(const HostPattern& F, const Schedule& s);

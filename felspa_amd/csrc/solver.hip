@@ -172,11 +172,14 @@ HostPattern pattern_of(const HostCsr& m) {
 
 // ---------------------------------------------------------------------------
 // ILU factor on the device
+static size_t dense_wg_lds_limit();
+
 void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k, int tsz_max) {
   sched = sc;
   n = src_pattern_new.n;
-  HostPattern F = symbolic_ilu(src_pattern_new, k, &sched);
-  if (!schedule_is_valid(F, sched))
+  bool conflict = false;
+  HostPattern F = symbolic_ilu(src_pattern_new, k, &sched, &conflict);
+  if (conflict || !schedule_is_valid(F, sched))
     throw FsError(FS_ERR_UNSUPPORTED, "ILU fill joins two blocks of one stage; use a larger block size");
   nnz = F.nnz();
   std::vector<int32_t> ol(n), od(n), ou(n);
@@ -207,11 +210,27 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       max_len = std::max<int>(max_len, (int)len);
     }
   }
-  // longest row of every stage: the numeric kernel's LDS per launch
+  // per stage: the longest row and L part (the numeric kernel's LDS), and
+  // the longest U part of any row up to the stage's end (the rows its L
+  // columns can name: the numeric kernel's loads in flight)
   stage_max_len.assign(sched.n_stages(), 1);
-  for (int s = 0; s < sched.n_stages(); ++s)
-    for (int32_t r = sched.blk_ptr[sched.stage_ptr[s]]; r < sched.blk_ptr[sched.stage_ptr[s + 1]]; ++r)
-      stage_max_len[s] = std::max<int>(stage_max_len[s], (int)(F.rp[r + 1] - F.rp[r]));
+  stage_max_nl.assign(sched.n_stages(), 0);
+  stage_max_u.assign(sched.n_stages(), 0);
+  {
+    int umax = 0;
+    for (int s = 0; s < sched.n_stages(); ++s) {
+      for (int32_t r = sched.blk_ptr[sched.stage_ptr[s]]; r < sched.blk_ptr[sched.stage_ptr[s + 1]]; ++r) {
+        const int len = (int)(F.rp[r + 1] - F.rp[r]);
+        stage_max_len[s] = std::max(stage_max_len[s], len);
+        stage_max_nl[s] = std::max(stage_max_nl[s], od[r]);
+        umax = std::max(umax, len - od[r] - 1);
+      }
+      stage_max_u[s] = umax;
+      int hs = 0, win = 0;
+      if (ilu_numeric_plan(stage_max_len[s], stage_max_nl[s], dense_wg_lds_limit(), hs, win) == 0)
+        throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel's LDS");
+    }
+  }
   for (int64_t i = 0; i < n; ++i) {
     lc[i + 1] += lc[i];
     uc[i + 1] += uc[i];
@@ -504,25 +523,29 @@ static int grid_for(int64_t n, int per_block = BLOCK, int cap = 4096) {
 void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit,
                        hipStream_t st, int* d_err, Counters& cnt) {
   IluView<double> V = view<double>();
-  if (ilu_numeric_lds_bytes(max_len) > 160 * 1024)
-    throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel");
   for (int s = 0; s < sched.n_stages(); ++s) {
     const int b0 = sched.stage_ptr[s], b1 = sched.stage_ptr[s + 1];
     if (b1 <= b0) continue;
-    // LDS sized by the stage's longest row; one wave per block for short
-    // rows, four for long ones
+    // LDS sized by the stage's longest row; threads x loads per thread cover
+    // the longest U part its L columns can name (one wave up to 256 entries,
+    // four up to 1024, eight waves of 8 loads each beyond)
     const int ml = stage_max_len[s];
-    const size_t lds = ilu_numeric_lds_bytes(ml);
-    const bool wide = ml > 128;
-    const void* fn = wide ? (const void*)k_ilu_numeric_stage<256, 2> : (const void*)k_ilu_numeric_stage<64, 4>;
-    if (lds > 64 * 1024)
-      hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), "hipFuncSetAttribute");
-    if (wide)
-      hipLaunchKernelGGL((k_ilu_numeric_stage<256, 2>), dim3(b1 - b0), dim3(256), lds, st, V, b0, b1, srp, sci, sv,
-                         col_limit, ml, d_err);
-    else
-      hipLaunchKernelGGL((k_ilu_numeric_stage<64, 4>), dim3(b1 - b0), dim3(64), lds, st, V, b0, b1, srp, sci, sv,
-                         col_limit, ml, d_err);
+    int hs = 0, win = 0;
+    const size_t lds = ilu_numeric_plan(ml, stage_max_nl[s], dense_wg_lds_limit(), hs, win);
+    if (lds == 0) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel's LDS");
+    const int um = stage_max_u[s];
+#define FS_NUM(NT, M)                                                                                          \
+  {                                                                                                            \
+    auto fn = &k_ilu_numeric_stage<NT, M>;                                                                     \
+    if (lds > 64 * 1024)                                                                                       \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(b1 - b0), dim3(NT), lds, st, V, b0, b1, srp, sci, sv, col_limit, ml, hs, win, d_err); \
+  }
+    if (um <= 256) FS_NUM(64, 4)
+    else if (um <= 1024) FS_NUM(256, 4)
+    else FS_NUM(512, 8)
+#undef FS_NUM
     cnt.launches++;
   }
   cnt.bytes += (double)nnz * 12.0 * 2.0;

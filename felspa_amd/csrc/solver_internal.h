@@ -84,7 +84,7 @@ struct IluDev {
   DevBuf lcp, lrow, lpos, lv64, lv32, ucp, urow, upos, uv64, uv32;
   int64_t n_lin = 0, n_uin = 0;
   int max_rows = 1, cap = 0, max_len = 1, G = 16;
-  std::vector<int> stage_max_len;  // longest row of every stage
+  std::vector<int> stage_max_len, stage_max_nl, stage_max_u;  // per stage: longest row, L part, named U part
   // explicit inverses of the in-block triangles (packed per block), used by
   // apply(); `dense` off keeps the in-block substitution kernels
   bool dense = true;

@@ -126,3 +126,35 @@ def test_block_color_schedule_is_valid(cfg, n, k, nodes):
     # or later stages (upper part): the forward sweep needs only finished stages
     lower = F.col < F.row
     assert np.all(stage_of[bj[lower & (bi != bj)]] < stage_of[bi[lower & (bi != bj)]])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_block_color_schedule_nonsymmetric_k2(seed):
+    """A structurally nonsymmetric pattern at level of fill 2: a block can be
+    reached from blocks it does not reach itself, so the colouring must treat
+    the coupling in both directions. No ILU(2) entry of the permuted matrix
+    may join two blocks of one stage, and the stage-parallel symbolic phase
+    that runs on that schedule must not meet such a pair."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(100 + seed)
+    m = 600
+    rows, cols = [np.arange(m)], [np.arange(m)]
+    for _ in range(3):  # one-directional couplings only
+        r = rng.integers(0, m, m)
+        c = (r + rng.integers(1, 40, m)) % m
+        rows.append(r)
+        cols.append(c)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    A = sp.csr_matrix((np.ones(len(r)), (r, c)), shape=(m, m))
+    A.sum_duplicates()
+    A.sort_indices()
+    assert (A != A.T).nnz > 0
+    perm, blk, stg = fa.block_color_ordering(A, 2, 8, None)
+    B = A[perm][:, perm].tocsr()
+    B.sort_indices()
+    F = fa.ilu_symbolic(B, 2).tocoo()
+    block_of = np.repeat(np.arange(len(blk) - 1), np.diff(blk))
+    stage_of = np.repeat(np.arange(len(stg) - 1), np.diff(stg))
+    bi, bj = block_of[F.row], block_of[F.col]
+    assert not np.any((bi != bj) & (stage_of[bi] == stage_of[bj]))

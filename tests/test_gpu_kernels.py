@@ -118,3 +118,60 @@ def test_ilu_zero_pivot_reported():
     with pytest.raises(L.FsException) as e:
         ilu.initialize(A, fa.SparseILU.AdditionalData(level_of_fill=0))
     assert e.value.code == L.FS_ERR_ZERO_PIVOT
+
+
+@pytest.mark.parametrize("cfg,n,hint", [("C5", 4, 3), ("C5", 6, 3), ("C3", 4, 3), ("C2", 16, 5)])
+def test_ilu2_block_color(oracle, cfg, n, hint):
+    """Level of fill 2 in the product path (block-colour order, lattice
+    blocks, dense stage kernels with split pieces, the numeric kernel's
+    windowed L columns): the factor is the reference SparseILU of P A P^T,
+    pattern bit for bit and values to 1e-12; FP64 applications to 1e-12 and
+    FP32 ones to 1e-4 of the oracle."""
+    S = system(cfg, n)
+    A = S.blocks["A"]
+    ilu = fa.SparseILU(ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=S.block_hint(hint))
+    ilu.initialize(A, fa.SparseILU.AdditionalData(level_of_fill=2))
+    F, perm = ilu.factors()
+    assert sorted(perm) == list(range(A.shape[0]))
+    PA = _perm_matrix(A, perm)
+    ref = oracle.SparseILU(PA, 2)
+    R = ref.factors()
+    assert (F.indptr == R.indptr).all() and (F.indices == R.indices).all()
+    assert np.abs(F.data - R.data).max() <= 1e-12 * np.abs(R.data).max()
+    rng = np.random.default_rng(5)
+    for trial in range(2):
+        b = rng.standard_normal(A.shape[0])
+        want = np.empty_like(b)
+        want[perm] = ref.vmult(b[perm])
+        assert rel(ilu.vmult(b), want) < 1e-12
+        assert rel(ilu.vmult(b, precision=L.FS_PREC_MIXED), want) < 1e-4
+    # refactorisation with new values on the same pattern
+    A2 = A.copy()
+    A2.data = A.data * (1.0 + 0.1 * rng.random(A.nnz))
+    A2 = (A2 + A2.T) * 0.5
+    A2 = A2.tocsr()
+    A2.sort_indices()
+    if A2.nnz == A.nnz:
+        ilu.initialize(A2, fa.SparseILU.AdditionalData(use_previous_sparsity=True, level_of_fill=2))
+        F2, _ = ilu.factors()
+        R2 = oracle.SparseILU(_perm_matrix(A2, perm), 2).factors()
+        assert np.abs(F2.data - R2.data).max() <= 1e-12 * np.abs(R2.data).max()
+
+
+def test_ilu_oversize_row_unsupported():
+    """A factor row longer than the numeric kernel's LDS plan (an arrow
+    matrix: the last row and column full) is refused at the symbolic phase
+    with FS_ERR_UNSUPPORTED, before any kernel runs."""
+    import scipy.sparse as sp
+    m = 9000
+    r = np.concatenate([np.arange(m), np.full(m - 1, m - 1), np.arange(m - 1)])
+    c = np.concatenate([np.arange(m), np.arange(m - 1), np.full(m - 1, m - 1)])
+    v = np.concatenate([np.full(m, 4.0), np.full(2 * (m - 1), 1e-3)])
+    A = sp.csr_matrix((v, (r, c)), shape=(m, m))
+    A.sort_indices()
+    hint = np.arange(m, dtype=np.int32) // 64
+    hint[-1] = m  # the arrow row in a block of its own, coloured last
+    ilu = fa.SparseILU(ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=hint)
+    with pytest.raises(L.FsException) as e:
+        ilu.initialize(A, fa.SparseILU.AdditionalData(level_of_fill=2))
+    assert e.value.code == L.FS_ERR_UNSUPPORTED
