@@ -564,15 +564,33 @@ static __global__ __launch_bounds__(BLOCK) void k_gather_vals(int64_t n, const i
   }
 }
 
-// d64[i] = src[pos[i]] (0 where pos[i] < 0: padding), d32 = (float) d64
-static __global__ __launch_bounds__(BLOCK) void k_gather_vals_pad(int64_t n, const int64_t* __restrict__ pos,
-                                                                  const double* __restrict__ src,
-                                                                  double* __restrict__ d64, float* __restrict__ d32) {
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-    const int64_t p = pos[i];
-    const double v = p >= 0 ? src[p] : 0.0;
-    d64[i] = v;
-    d32[i] = (float)v;
+// External entries of one triangle of an ILU factor, block after block in
+// chunks of 4 (the dense stage kernels' layout): block b = desc[b] {first
+// row, rows, first chunk, chunks}; row r's entries start at chunk first +
+// (rng[r] & 0xffff) and fill rng[r] >> 16 chunks. Its entries are the factor
+// row's positions [0, o_lint) (lower) or [o_uext, len) (upper), in order,
+// then zeros up to the chunk count. FP64 and FP32 copies.
+static __global__ __launch_bounds__(BLOCK) void k_gather_ext(const i32x4* __restrict__ desc,
+                                                             const int32_t* __restrict__ rng,
+                                                             const int64_t* __restrict__ rp,
+                                                             const int32_t* __restrict__ o_lint,
+                                                             const int32_t* __restrict__ o_uext, int lower,
+                                                             const double* __restrict__ v, double* __restrict__ x64,
+                                                             float* __restrict__ x32, int nb) {
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const i32x4 d = desc[b];
+    for (int r = d.x; r < d.x + d.y; ++r) {
+      const uint32_t g = (uint32_t)rng[r];
+      const int64_t dst = 4 * ((int64_t)d.z + (g & 0xffffu));
+      const int ne = 4 * (int)(g >> 16);
+      const int64_t src = rp[r] + (lower ? 0 : o_uext[r]);
+      const int cnt = lower ? o_lint[r] : (int)(rp[r + 1] - rp[r]) - o_uext[r];
+      for (int e = threadIdx.x; e < ne; e += BLOCK) {
+        const double val = e < cnt ? v[src + e] : 0.0;
+        x64[dst + e] = val;
+        x32[dst + e] = (float)val;
+      }
+    }
   }
 }
 
@@ -1327,7 +1345,8 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
                                                           const int64_t* __restrict__ srp,
                                                           const int32_t* __restrict__ sci,
                                                           const double* __restrict__ sv, int64_t col_limit,
-                                                          int max_len, int hs, int win, int* __restrict__ err) {
+                                                          int max_len, int hs, int win, int from_factor,
+                                                          int* __restrict__ err) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   const uint32_t mask = (uint32_t)hs - 1;
   const int lg = __builtin_ctz((uint32_t)hs);
@@ -1346,12 +1365,16 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     const int64_t base = F.rp[r];
     const int len = (int)(F.rp[r + 1] - base);
     const int nl = F.o_diag[r];
+    // from_factor: the external L columns were applied by the stage updates
+    // and the row's values are in the factor; only the in-block L columns
+    // (positions [o_lint, o_diag)) remain
+    const int t0 = from_factor ? F.o_lint[r] : 0;
     for (int h = tid; h < hs; h += NT) tab[h] = NUM_EMPTY;
     lds_barrier();
     for (int t = tid; t < len; t += NT) {
       const int32_t c = F.ci[base + t];
       lc[t] = c;
-      w[t] = 0.0;
+      w[t] = from_factor && t >= t0 ? F.v[base + t] : 0.0;
       const uint32_t hh = num_hash(c);
       const uint32_t ent = ((hh & 0xFFFFu) << 16) | (uint32_t)t;
       for (uint32_t h = hh >> (32 - lg);; h = (h + 1) & mask)
@@ -1361,12 +1384,12 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     // the U part and pivot of every L column of window wi
     auto load_window = [&](int wi) {
       for (int j = tid; j < win; j += NT) {
-        const int t = wi * win + j;
+        const int t = t0 + wi * win + j;
         if (t < nl) {
           const int32_t k = lc[t];
           const int64_t kb = F.rp[k];
           const int kd = F.o_diag[k];
-          const int s = t & rmask;
+          const int s = (t - t0) & rmask;
           mq[s] = kb + kd + 1;
           mn[s] = (int)(F.rp[k + 1] - kb - kd - 1);
           mu[s] = F.v[kb + kd];
@@ -1375,12 +1398,13 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     };
     load_window(0);
     load_window(1);
-    for (int64_t p = srp[r] + tid; p < srp[r + 1]; p += NT) {
-      const int32_t c = sci[p];
-      if (c >= col_limit) continue;
-      const int pos = num_find(tab, lc, mask, lg, c);
-      if (pos >= 0) w[pos] = sv[p];
-    }
+    if (!from_factor)
+      for (int64_t p = srp[r] + tid; p < srp[r + 1]; p += NT) {
+        const int32_t c = sci[p];
+        if (c >= col_limit) continue;
+        const int pos = num_find(tab, lc, mask, lg, c);
+        if (pos >= 0) w[pos] = sv[p];
+      }
     lds_barrier();
     int loaded = 1;  // last window in the ring
     // three register sets in rotation: the U part of L column t + 3 loads
@@ -1392,7 +1416,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     // set's use leave the two younger sets in flight)
     auto fetch = [&](int t, int32_t* j, double* v) {
       const bool ok = t < nl;
-      const int s = (ok ? t : 0) & rmask;
+      const int s = ((ok ? t : t0) - t0) & rmask;
       const int64_t q = mq[s];
       const int n = ok ? mn[s] : 0;
 #pragma unroll
@@ -1428,7 +1452,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     };
     auto apply = [&](int t, const int32_t* j, const double* v) {
       // w[t] is final: later L columns only update positions > t
-      const int s = t & rmask;
+      const int s = (t - t0) & rmask;
       const double mult = w[t] / mu[s];
       const int n = mn[s];
       if (tid == 0) F.v[base + t] = mult;  // l_it is final (no row reads row r's L part)
@@ -1453,15 +1477,15 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     // applied or fetched lies in the current window or later (win >= 64, so
     // the fetches up to t + 5 never reach the window being loaded)
     auto refill = [&](int t) {
-      if (t >= loaded * win && (loaded + 1) * win < nl) {
+      if (t - t0 >= loaded * win && (loaded + 1) * win < nl - t0) {
         ++loaded;
         load_window(loaded);
       }
     };
-    fetch(0, ja, va);
-    fetch(1, jb, vb);
-    fetch(2, jc, vc);
-    int t = 0;
+    fetch(t0, ja, va);
+    fetch(t0 + 1, jb, vb);
+    fetch(t0 + 2, jc, vc);
+    int t = t0;
     for (; t + 2 < nl; t += 3) {  // (no exit inside the body: every path
                                   // into the loop has the same loads in flight)
       refill(t);
@@ -1490,6 +1514,216 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
     }
     __syncthreads();  // the row's values are read by the block's later rows
   }
+}
+
+// Left-looking ILU by stages. Every external L column of a row (a column in
+// another block) lies in an earlier stage, and in IKJ order a row applies its
+// L columns in ascending order: all external ones, stage by stage, then the
+// in-block ones. So once the rows of stage s are final, every later row can
+// apply its L columns of stage s at once, with its partial values kept in the
+// factor between launches; the per-block kernel above (from_factor) then
+// applies the in-block columns and forms the pivots. Each entry receives the
+// same updates in the same order as in the row-by-row IKJ loop.
+//
+// Source values first: the factor zeroed, then every entry of the source
+// row (columns < col_limit) stored at its position (binary search in the
+// factor row). One wave per row.
+static __global__ __launch_bounds__(256) void k_ilu_source_scatter(IluView<double> F, int64_t n,
+                                                            const int64_t* __restrict__ srp,
+                                                            const int32_t* __restrict__ sci,
+                                                            const double* __restrict__ sv, int64_t col_limit) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int64_t base = F.rp[i];
+  const int len = (int)(F.rp[i + 1] - base);
+  for (int64_t p = srp[i] + lane; p < srp[i + 1]; p += 64) {
+    const int32_t c = sci[p];
+    if (c >= col_limit) continue;
+    int lo = 0, hi = len;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (F.ci[base + mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < len && F.ci[base + lo] == c) F.v[base + lo] = sv[p];
+  }
+}
+
+// first position p in [0, n) with a[p] >= key (a ascending), by the 64 lanes
+// of a wave: probes spread over the range narrow it to 64 entries, then one
+// exact round; every lane returns the same value
+__device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ a, int n, int32_t key, int lane) {
+  int lo = 0, hi = n;
+  while (hi - lo > 64) {
+    const int step = (hi - lo + 63) >> 6;
+    const int p = lo + lane * step;
+    const bool lt = p < hi && a[p] < key;
+    const int cnt = __popcll(__ballot(lt));
+    if (cnt == 0) {
+      hi = lo;
+      break;
+    }
+    const int nlo = lo + (cnt - 1) * step + 1;
+    hi = min(hi, lo + cnt * step);
+    lo = nlo;
+  }
+  const int p = lo + lane;
+  const bool lt = p < hi && a[p] < key;
+  return lo + __popcll(__ballot(lt));
+}
+
+// LDS of one stage-update workgroup: the row suffix's working values (and a
+// dump slot), a batch of 64 L columns' U spans and pivots, the suffix's
+// columns and the hash map
+constexpr int UPD_BATCH = 64;
+__host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs) {
+  return (size_t)(mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 4) + (size_t)mcap * 4 + (size_t)hs * 4;
+}
+
+// One stage update: rows[0 .. nrows) (ascending, each of at most mcap
+// entries, all after the stage) apply their external L columns in
+// [col_a, col_b), the rows of the finished stage. One workgroup of NT
+// threads per row, workgroups XCD-contiguous so that neighbouring rows, which
+// name the same finished rows, share an L2. The row from its first such
+// column on goes to LDS (values, columns, hash map); then for the L columns
+// in order, in batches of 64 whose U spans and pivots are read first: m =
+// w_t / u_kk goes to the factor, and the U part of row k (loaded three
+// columns ahead, M entries per thread per round) updates w_j -= m u_kj where
+// row i stores j. Then the rest of the suffix goes back to the factor.
+template <int NT, int M>
+__global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
+                                                           int nrows, int32_t col_a, int32_t col_b, int mcap,
+                                                           int hs) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  double* w = (double*)smem;                  // [mcap + 1]; w[mcap]: dump slot
+  int64_t* mq = (int64_t*)(w + mcap + 1);     // [UPD_BATCH] first U entry of row k
+  double* mu = (double*)(mq + UPD_BATCH);     // [UPD_BATCH] pivot u_kk
+  int32_t* mn = (int32_t*)(mu + UPD_BATCH);   // [UPD_BATCH] U entries of row k
+  int32_t* lc = mn + UPD_BATCH;               // [mcap] columns of the suffix
+  uint32_t* tab = (uint32_t*)(lc + mcap);     // [hs] hash map
+  const uint32_t mask = (uint32_t)hs - 1;
+  const int lg = __builtin_ctz((uint32_t)hs);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int item = xcd_contiguous(blockIdx.x, gridDim.x);
+  if (item >= nrows) return;
+  const int32_t i = rows[item];
+  const int64_t base = F.rp[i];
+  const int len = (int)(F.rp[i + 1] - base);
+  const int nx = F.o_lint[i];
+  // (every wave finds the same range)
+  const int pa = wave_lower_bound(F.ci + base, nx, col_a, lane);
+  const int pb = wave_lower_bound(F.ci + base, nx, col_b, lane);
+  if (pa == pb) return;  // the whole workgroup: no barrier is passed
+  const int m = len - pa;  // positions [pa, len): every column a U part of stage rows can name
+  const int nc = pb - pa;
+  for (int h = tid; h < hs; h += NT) tab[h] = NUM_EMPTY;
+  lds_barrier();
+  for (int u = tid; u < m; u += NT) {
+    const int32_t c = F.ci[base + pa + u];
+    lc[u] = c;
+    w[u] = F.v[base + pa + u];
+    const uint32_t hh = num_hash(c);
+    const uint32_t ent = ((hh & 0xFFFFu) << 16) | (uint32_t)u;
+    for (uint32_t h = hh >> (32 - lg);; h = (h + 1) & mask)
+      if (atomicCAS(&tab[h], NUM_EMPTY, ent) == NUM_EMPTY) break;
+  }
+  lds_barrier();
+  int32_t ja[M], jb[M], jc[M];
+  double va[M], vb[M], vc[M];
+  for (int b0 = 0; b0 < nc; b0 += UPD_BATCH) {
+    const int nb = min(UPD_BATCH, nc - b0);
+    if (tid < nb) {
+      const int32_t k = lc[b0 + tid];
+      const int64_t kb = F.rp[k];
+      const int kd = F.o_diag[k];
+      mq[tid] = kb + kd + 1;
+      mn[tid] = (int)(F.rp[k + 1] - kb - kd - 1);
+      mu[tid] = F.v[kb + kd];
+    }
+    lds_barrier();
+    // (every load is issued, from entry 0 when out of range, so every path
+    // has the same loads in flight)
+    auto fetch = [&](int c, int32_t* j, double* v) {
+      const bool ok = c < nb;
+      const int s = ok ? c : 0;
+      const int64_t q = mq[s];
+      const int n = ok ? mn[s] : 0;
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int idx = tid + r * NT;
+        const int64_t a = idx < n ? q + idx : 0;
+        j[r] = F.ci[a];
+        v[r] = F.v[a];
+      }
+    };
+    auto update = [&](double mult, int n, int off, const int32_t* j, const double* v) {
+      uint32_t h[M], e[M];
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        h[r] = num_hash(j[r]) >> (32 - lg);
+        e[r] = tab[h[r]];
+      }
+      int pos[M];
+#pragma unroll
+      for (int r = 0; r < M; ++r) {
+        const int p = num_probe(tab, lc, mask, j[r], num_hash(j[r]) & 0xFFFFu, h[r], e[r]);
+        pos[r] = off + tid + r * NT < n && p >= 0 ? p : mcap;
+      }
+      double o[M];
+#pragma unroll
+      for (int r = 0; r < M; ++r) o[r] = w[pos[r]];
+#pragma unroll
+      for (int r = 0; r < M; ++r) w[pos[r]] = o[r] - mult * v[r];
+    };
+    auto apply = [&](int c, const int32_t* j, const double* v) {
+      // w at column b0 + c is final: earlier columns are applied, later
+      // ones only update positions to its right
+      const double mult = w[b0 + c] / mu[c];
+      if (tid == 0) F.v[base + pa + b0 + c] = mult;
+      const int n = mn[c];
+      update(mult, n, 0, j, v);
+      if (n > M * NT) {  // U parts longer than M NT: further rounds
+        const int64_t q = mq[c];
+        for (int off = M * NT; off < n; off += M * NT) {
+          int32_t jt[M];
+          double vt[M];
+#pragma unroll
+          for (int r = 0; r < M; ++r) {
+            const int idx = off + tid + r * NT;
+            const int64_t a = idx < n ? q + idx : 0;
+            jt[r] = F.ci[a];
+            vt[r] = F.v[a];
+          }
+          update(mult, n, off, jt, vt);
+        }
+      }
+    };
+    fetch(0, ja, va);
+    fetch(1, jb, vb);
+    fetch(2, jc, vc);
+    int c = 0;
+    for (; c + 2 < nb; c += 3) {
+      apply(c, ja, va);
+      fetch(c + 3, ja, va);
+      lds_barrier();
+      apply(c + 1, jb, vb);
+      fetch(c + 4, jb, vb);
+      lds_barrier();
+      apply(c + 2, jc, vc);
+      fetch(c + 5, jc, vc);
+      lds_barrier();
+    }
+    if (c < nb) {
+      apply(c, ja, va);
+      lds_barrier();
+    }
+    if (c + 1 < nb) {
+      apply(c + 1, jb, vb);
+      lds_barrier();
+    }
+  }
+  for (int u = nc + tid; u < m; u += NT) F.v[base + pa + u] = w[u];
 }
 
 // ---------------------------------------------------------------------------

@@ -231,6 +231,43 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel's LDS");
     }
   }
+  // rows longer than 512 entries (3D): factorise by stages, the external L
+  // columns of all later rows applied after each stage
+  bins.clear();
+  staged = max_len > 512 && sched.n_stages() > 1;
+  if (staged) {
+    const size_t lim = dense_wg_lds_limit();
+    static const int caps[] = {256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096, 5120, 6144, 8192};
+    std::vector<std::vector<int32_t>> lists(sizeof(caps) / sizeof(caps[0]));
+    for (int64_t r = 0; r < n; ++r) {
+      if (ol[r] == 0) continue;
+      const int len = (int)(F.rp[r + 1] - F.rp[r]);
+      size_t b = 0;
+      while (b + 1 < lists.size() && caps[b] < len) ++b;
+      if (caps[b] < len) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
+      lists[b].push_back((int32_t)r);
+    }
+    for (size_t b = 0; b < lists.size(); ++b) {
+      if (lists[b].empty()) continue;
+      NumBin nb;
+      nb.mcap = caps[b];
+      // hash load factor <= 1/4 unless that costs waves per CU, never above 1/2
+      int hmin = 64, hgood = 64;
+      while (hmin < 2 * nb.mcap) hmin *= 2;
+      while (hgood < 4 * nb.mcap) hgood *= 2;
+      const size_t wmin = ilu_update_lds_bytes(nb.mcap, hmin), wgood = ilu_update_lds_bytes(nb.mcap, hgood);
+      if (wmin > lim) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
+      nb.hs = (wgood <= lim && lim / wgood >= lim / wmin) ? hgood : hmin;
+      nb.nrows = (int)lists[b].size();
+      nb.rows.upload(lists[b].data(), 4 * lists[b].size());
+      nb.first.resize(sched.n_stages());
+      for (int st = 0; st < sched.n_stages(); ++st) {
+        const int32_t after = sched.blk_ptr[sched.stage_ptr[st + 1]];
+        nb.first[st] = (int)(std::lower_bound(lists[b].begin(), lists[b].end(), after) - lists[b].begin());
+      }
+      bins.push_back(std::move(nb));
+    }
+  }
   for (int64_t i = 0; i < n; ++i) {
     lc[i + 1] += lc[i];
     uc[i + 1] += uc[i];
@@ -327,7 +364,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     // block after each other. Per block {first row, rows, first chunk,
     // chunks}; per row its first chunk relative to the block (low 16 bits)
     // and its chunk count (high 16 bits).
-    auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& didx, DevBuf& dhcol, DevBuf& dhalo, DevBuf& dpos,
+    auto ext = [&](bool lower, DevBuf& desc, DevBuf& rng, DevBuf& didx, DevBuf& dhcol, DevBuf& dhalo,
                    DevBuf& dv64, DevBuf& dv32, int64_t& n_ent, int& c_max, std::vector<int32_t>& dsc, int& ib,
                    int& h_max, int& ph_max, int64_t& n_halo, std::vector<std::vector<int32_t>>& phl) {
       const int nb = sched.n_blocks();
@@ -353,92 +390,106 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         c_max = std::max<int>(c_max, (int)(q - qb));
       }
       n_ent = 4 * q;
-      std::vector<int32_t> cc(std::max<int64_t>(4, n_ent));
-      std::vector<int64_t> po(std::max<int64_t>(4, n_ent), -1);
-      int64_t e = 0;
-      for (int b = 0; b < nb; ++b)
-        for (int32_t r = sched.blk_ptr[b]; r < sched.blk_ptr[b + 1]; ++r) {
-          const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
-          const int64_t p0 = lower ? 0 : ou[r], p1 = lower ? ol[r] : len;
-          const int64_t e0 = e;
-          for (int64_t p = p0; p < p1; ++p, ++e) {
-            cc[e] = F.ci[base + p];
-            po[e] = base + p;
-          }
-          for (; (e - e0) % 4; ++e) cc[e] = cc[e0];
-        }
       // the block's halo: its distinct external columns, ascending (nearby
       // lanes gather nearby values); every entry stores its column's index in
       // the halo (1 byte when no halo exceeds 256 columns, else 2); halos
-      // start at multiples of 4. A block split into pieces (more than 2
-      // DENSE_SPLIT chunks) has a halo per piece instead, listed in phl.
-      std::vector<int32_t> hc, hl(2 * (size_t)nb, 0), hb;
-      std::vector<int32_t> loc(cc.size(), 0), stamp(n, -1), slot(n, 0);
-      phl.assign(nb, {});
-      h_max = 0;
-      int stamp_id = 0;
-      auto halo = [&](int64_t a, int64_t z, int32_t& off, int32_t& cnt) {
-        hb.clear();
-        ++stamp_id;
-        for (int64_t e2 = a; e2 < z; ++e2)
-          if (stamp[cc[e2]] != stamp_id) {
-            stamp[cc[e2]] = stamp_id;
-            hb.push_back(cc[e2]);
+      // start at multiples of 4. A block of more than DENSE_SPLIT_ABOVE chunks
+      // has a halo per piece of DENSE_SPLIT chunks instead, listed in phl.
+      // Padding entries repeat the row's first external column. Blocks are
+      // independent: halos are formed in parallel, then laid end to end.
+      std::vector<uint16_t> loc(std::max<int64_t>(4, n_ent), 0);
+      std::vector<std::vector<int32_t>> bh(nb);   // per block: its halo(s), padded
+      std::vector<std::vector<int32_t>> bpc(nb);  // per block: {offset in bh, count} per halo
+      int too_big = 0, hm = 0, phm = 0;
+#pragma omp parallel reduction(| : too_big) reduction(max : hm, phm)
+      {
+        std::vector<int32_t> stamp(n, -1), slot(n, 0), cc, hb;
+#pragma omp for schedule(dynamic, 16)
+        for (int b = 0; b < nb; ++b) {
+          const int nc = dsc[4 * b + 3];
+          const int64_t a = 4 * (int64_t)dsc[4 * b + 2];
+          cc.clear();
+          for (int32_t r = sched.blk_ptr[b]; r < sched.blk_ptr[b + 1]; ++r) {
+            const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
+            const int64_t p0 = lower ? 0 : ou[r], p1 = lower ? ol[r] : len;
+            const size_t e0 = cc.size();
+            for (int64_t p = p0; p < p1; ++p) cc.push_back(F.ci[base + p]);
+            while ((cc.size() - e0) % 4) cc.push_back(cc[e0]);
           }
-        std::sort(hb.begin(), hb.end());
-        for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
-        for (int64_t e2 = a; e2 < z; ++e2) loc[e2] = slot[cc[e2]];
-        if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external columns");
-        off = (int32_t)hc.size();
-        cnt = (int32_t)hb.size();
-        hc.insert(hc.end(), hb.begin(), hb.end());
-        while (hc.size() % 4) hc.push_back(hb.empty() ? 0 : hb[0]);
-      };
-      for (int b = 0; b < nb; ++b) {
-        const int nc = dsc[4 * b + 3];
-        const int64_t a = 4 * (int64_t)dsc[4 * b + 2], z = a + 4 * (int64_t)nc;
-        if (nc > DENSE_SPLIT_ABOVE) {
-          for (int c = 0; c < nc; c += DENSE_SPLIT) {
-            int32_t off = 0, cnt = 0;
-            halo(a + 4 * (int64_t)c, a + 4 * (int64_t)std::min(nc, c + DENSE_SPLIT), off, cnt);
-            phl[b].push_back(off);
-            phl[b].push_back(cnt);
-            ph_max = std::max<int>(ph_max, cnt);
+          const bool split = nc > DENSE_SPLIT_ABOVE;
+          for (int c = 0; c < (split ? nc : std::min(nc, 1)); c += split ? DENSE_SPLIT : nc) {
+            const size_t ea = 4 * (size_t)c, ez = 4 * (size_t)(split ? std::min(nc, c + DENSE_SPLIT) : nc);
+            hb.clear();
+            for (size_t e2 = ea; e2 < ez; ++e2)
+              if (stamp[cc[e2]] != b) {
+                stamp[cc[e2]] = b;
+                hb.push_back(cc[e2]);
+              }
+            for (size_t e2 = ea; e2 < ez; ++e2) stamp[cc[e2]] = -1;
+            std::sort(hb.begin(), hb.end());
+            if (hb.size() > 65536) {
+              too_big = 1;
+              continue;
+            }
+            for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
+            for (size_t e2 = ea; e2 < ez; ++e2) loc[a + e2] = (uint16_t)slot[cc[e2]];
+            bpc[b].push_back((int32_t)bh[b].size());
+            bpc[b].push_back((int32_t)hb.size());
+            bh[b].insert(bh[b].end(), hb.begin(), hb.end());
+            while (bh[b].size() % 4) bh[b].push_back(hb.empty() ? 0 : hb[0]);
+            if (split) phm = std::max<int>(phm, (int)hb.size());
+            else hm = std::max<int>(hm, (int)hb.size());
           }
-        } else {
-          halo(a, z, hl[2 * b], hl[2 * b + 1]);
-          h_max = std::max<int>(h_max, hl[2 * b + 1]);
         }
       }
-      if (hc.size() > (size_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external columns");
-      if (hc.empty()) hc.assign(4, 0);
+      if (too_big) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external columns");
+      h_max = hm;
+      ph_max = std::max(ph_max, phm);
+      std::vector<int64_t> hoff(nb + 1, 0);
+      for (int b = 0; b < nb; ++b) hoff[b + 1] = hoff[b] + (int64_t)bh[b].size();
+      if (hoff[nb] > (int64_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "ILU factor with too many external columns");
+      std::vector<int32_t> hc(std::max<int64_t>(4, hoff[nb]), 0), hl(2 * (size_t)nb, 0);
+      phl.assign(nb, {});
+      for (int b = 0; b < nb; ++b) {
+        std::copy(bh[b].begin(), bh[b].end(), hc.begin() + hoff[b]);
+        const bool split = dsc[4 * b + 3] > DENSE_SPLIT_ABOVE;
+        for (size_t k2 = 0; k2 + 1 < bpc[b].size(); k2 += 2) {
+          const int32_t off = (int32_t)(hoff[b] + bpc[b][k2]), cnt = bpc[b][k2 + 1];
+          if (split) {
+            phl[b].push_back(off);
+            phl[b].push_back(cnt);
+          } else {
+            hl[2 * b] = off;
+            hl[2 * b + 1] = cnt;
+          }
+        }
+        std::vector<int32_t>().swap(bh[b]);
+      }
       n_halo = (int64_t)hc.size();
       ib = std::max(h_max, ph_max) <= 256 ? 1 : 2;
       if (ib == 1) {
         std::vector<uint8_t> ix(loc.size());
+#pragma omp parallel for schedule(static)
         for (size_t e2 = 0; e2 < loc.size(); ++e2) ix[e2] = (uint8_t)loc[e2];
         didx.upload(ix.data(), ix.size());
       } else {
-        std::vector<uint16_t> ix(loc.size());
-        for (size_t e2 = 0; e2 < loc.size(); ++e2) ix[e2] = (uint16_t)loc[e2];
-        didx.upload(ix.data(), 2 * ix.size());
+        didx.upload(loc.data(), 2 * loc.size());
       }
       desc.upload(dsc.data(), 4 * dsc.size());
       rng.upload(rg.data(), 4 * rg.size());
       dhcol.upload(hc.data(), 4 * hc.size());
       dhalo.upload(hl.data(), 4 * hl.size());
-      dpos.upload(po.data(), 8 * po.size());
-      dv64.alloc(8 * cc.size());
-      dv32.alloc(4 * cc.size());
+      dv64.alloc(8 * loc.size());
+      dv32.alloc(4 * loc.size());
     };
     n_ext = sum_lext + sum_uext;
     std::vector<int32_t> dl, du;
     std::vector<std::vector<int32_t>> pl, pu;
     xl_phmax = xu_phmax = 0;
-    ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, xl_phmax,
-        n_xlh, pl);
-    ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, xu_phmax,
-        n_xuh, pu);
+    ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, xl_phmax, n_xlh,
+        pl);
+    ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, xu_phmax, n_xuh,
+        pu);
     // blocks with many external chunks are summed in pieces of DENSE_SPLIT
     // chunks by workgroups of their own; pieces listed stage by stage
     int64_t part_need = 0;
@@ -490,7 +541,17 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       fuse_last = cnt == 0;
     }
   }
-  host_pattern = std::move(F);
+  // the FP32 copy of the factor in CSR form serves the substitution kernels
+  // only: not needed when the dense kernels take every precision
+  need_v32 = true;
+  if (dense) {
+    const size_t lim = dense_wg_lds_limit();
+    const int hc = (std::max(xl_hmax, xu_hmax) + 3) & ~3;
+    auto fits = [&](int tv, int t, int rc) { return dense_lds_bytes(inv_cap, hc, 512 * rc, tv, t, DS_FWD) <= lim; };
+    need_v32 = !(fits(8, 8, dense_rc<double, double>()) && fits(4, 8, dense_rc<float, double>()) &&
+                 fits(4, 4, dense_rc<float, float>()));
+  }
+  if (keep_pattern) host_pattern = std::move(F);
 }
 
 template <typename T>
@@ -523,6 +584,12 @@ static int grid_for(int64_t n, int per_block = BLOCK, int cap = 4096) {
 void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit,
                        hipStream_t st, int* d_err, Counters& cnt) {
   IluView<double> V = view<double>();
+  if (staged) {
+    hip_check(hipMemsetAsync(v64.p, 0, 8 * (size_t)nnz, st), "memset");
+    hipLaunchKernelGGL(k_ilu_source_scatter, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, V, n, srp, sci, sv,
+                       col_limit);
+    cnt.launches++;
+  }
   for (int s = 0; s < sched.n_stages(); ++s) {
     const int b0 = sched.stage_ptr[s], b1 = sched.stage_ptr[s + 1];
     if (b1 <= b0) continue;
@@ -540,29 +607,59 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     if (lds > 64 * 1024)                                                                                       \
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
                 "hipFuncSetAttribute");                                                                        \
-    hipLaunchKernelGGL(fn, dim3(b1 - b0), dim3(NT), lds, st, V, b0, b1, srp, sci, sv, col_limit, ml, hs, win, d_err); \
+    hipLaunchKernelGGL(fn, dim3(b1 - b0), dim3(NT), lds, st, V, b0, b1, srp, sci, sv, col_limit, ml, hs, win,   \
+                       staged ? 1 : 0, d_err);                                                                 \
   }
     if (um <= 256) FS_NUM(64, 4)
     else if (um <= 1024) FS_NUM(256, 4)
     else FS_NUM(512, 8)
 #undef FS_NUM
     cnt.launches++;
+    if (staged && s + 1 < sched.n_stages()) {
+      // the finished stage's columns into every later row
+      const int32_t ca = sched.blk_ptr[b0], cb = sched.blk_ptr[b1];
+      for (const NumBin& nb : bins) {
+        const int cnt_rows = nb.nrows - nb.first[s];
+        if (cnt_rows <= 0) continue;
+        // one wave per row where several rows share a CU's LDS, four where
+        // a row's LDS leaves room for one or two
+        const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs);
+        const int32_t* rl = nb.rows.as<int32_t>() + nb.first[s];
+#define FS_UPD(NT, M)                                                                                          \
+  {                                                                                                            \
+    auto fn = &k_ilu_numeric_update<NT, M>;                                                                    \
+    if (lds > 64 * 1024)                                                                                       \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, cnt_rows, ca, cb, nb.mcap, nb.hs);        \
+  }
+        if (nb.mcap <= 1024) FS_UPD(64, 8)
+        else FS_UPD(256, 4)
+#undef FS_UPD
+        cnt.launches++;
+      }
+    }
   }
   cnt.bytes += (double)nnz * 12.0 * 2.0;
   hip_check(hipGetLastError(), "ilu numeric launch");
-  if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz,
-                     (const double*)v64.p, (float*)v32.p);
-  cnt.launches++;
+  if (need_v32) {
+    if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz,
+                       (const double*)v64.p, (float*)v32.p);
+    cnt.launches++;
+  }
   gather_inblock(st);
   if (dense) {
-    if (n_xl > 0)
-      hipLaunchKernelGGL(k_gather_vals_pad, dim3(grid_for(n_xl)), dim3(BLOCK), 0, st, n_xl, xlpos.as<int64_t>(),
-                         v64.as<double>(), xlv64.as<double>(), xlv32.as<float>());
-    if (n_xu > 0)
-      hipLaunchKernelGGL(k_gather_vals_pad, dim3(grid_for(n_xu)), dim3(BLOCK), 0, st, n_xu, xupos.as<int64_t>(),
-                         v64.as<double>(), xuv64.as<double>(), xuv32.as<float>());
     const int nb = sched.n_blocks();
+    const int gb = std::max(1, std::min(nb, 65535));
+    if (n_xl > 0)
+      hipLaunchKernelGGL(k_gather_ext, dim3(gb), dim3(BLOCK), 0, st, xldesc.as<i32x4>(), xlrng.as<int32_t>(),
+                         rp.as<int64_t>(), o_lint.as<int32_t>(), o_uext.as<int32_t>(), 1, v64.as<double>(),
+                         xlv64.as<double>(), xlv32.as<float>(), nb);
+    if (n_xu > 0)
+      hipLaunchKernelGGL(k_gather_ext, dim3(gb), dim3(BLOCK), 0, st, xudesc.as<i32x4>(), xurng.as<int32_t>(),
+                         rp.as<int64_t>(), o_lint.as<int32_t>(), o_uext.as<int32_t>(), 0, v64.as<double>(),
+                         xuv64.as<double>(), xuv32.as<float>(), nb);
     hipLaunchKernelGGL(k_ilu_block_inverse, dim3(std::max(1, std::min(nb, 65535))), dim3(BLOCK), 0, st, V, nb,
                        inv_off.as<int64_t>(), linv64.as<double>(), uinv64.as<double>());
     hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(inv_total)), dim3(BLOCK), 0, st, inv_total,
@@ -758,6 +855,8 @@ void IluDev::apply(const TB* b, T* tmp, T* x, hipStream_t st, Counters& cnt, con
   }
   if (!std::is_same<TV, T>::value)
     throw FsError(FS_ERR_UNSUPPORTED, "ILU application with FP32 factors and FP64 vectors needs the dense kernels");
+  if (std::is_same<T, float>::value && !v32.p)
+    throw FsError(FS_ERR_UNSUPPORTED, "ILU substitution kernels without the FP32 factor copy");
   if (pf && pf->r) {
     hipLaunchKernelGGL((k_cg1_update<T>), dim3(256), dim3(BLOCK), 0, st, (int64_t)n, pf->alpha, pf->beta, pf->z,
                        pf->s, pf->p, pf->q, pf->x, pf->r, pf->first);

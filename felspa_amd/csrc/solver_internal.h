@@ -32,6 +32,20 @@ struct DevBuf {
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf();
   void reset();
   void alloc(size_t b);
@@ -85,6 +99,19 @@ struct IluDev {
   int64_t n_lin = 0, n_uin = 0;
   int max_rows = 1, cap = 0, max_len = 1, G = 16;
   std::vector<int> stage_max_len, stage_max_nl, stage_max_u;  // per stage: longest row, L part, named U part
+  // left-looking numeric factorisation by stages (rows with long external L
+  // parts): rows with external L columns in bins of length, each bin's rows
+  // ascending; first[s] = first row of the bin after stage s
+  struct NumBin {
+    int mcap = 0, hs = 0;
+    DevBuf rows;
+    std::vector<int> first;
+    int nrows = 0;
+  };
+  bool staged = false;
+  bool need_v32 = true;      // FP32 CSR copy of the factor (substitution kernels)
+  bool keep_pattern = true;  // keep the host pattern (factor queries, transposed factor)
+  std::vector<NumBin> bins;
   // explicit inverses of the in-block triangles (packed per block), used by
   // apply(); `dense` off keeps the in-block substitution kernels
   bool dense = true;
@@ -95,8 +122,8 @@ struct IluDev {
   int xl_max = 0, xu_max = 0;      // largest per-block count of chunks of external L / U entries
   bool fuse_last = false;          // last stage: forward and backward in one launch
   DevBuf inv_off, linv64, uinv64, linv32, uinv32;
-  DevBuf xldesc, xlrng, xlidx, xlhcol, xlhalo, xlpos, xlv64, xlv32;
-  DevBuf xudesc, xurng, xuidx, xuhcol, xuhalo, xupos, xuv64, xuv32;
+  DevBuf xldesc, xlrng, xlidx, xlhcol, xlhalo, xlv64, xlv32;
+  DevBuf xudesc, xurng, xuidx, xuhcol, xuhalo, xuv64, xuv32;
   int xl_ib = 1, xu_ib = 1;          // bytes per halo index of an external entry
   int xl_hmax = 0, xu_hmax = 0;      // largest halo of a block
   int xl_phmax = 0, xu_phmax = 0;    // largest halo of a piece of a split block

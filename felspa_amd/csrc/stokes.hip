@@ -1097,6 +1097,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   HostCsr Anew = permute_square(hA, sA.perm, sA.iperm, nullptr);
   std::vector<int64_t> mapM;
   HostCsr Mnew = permute_square(hM, sS.perm, sS.iperm, &mapM);
+  h.FA.keep_pattern = false;
   h.FA.build(pattern_of(Anew), sA, o.ilu_k_A, 8);
   h.FS.dense = false;  // M_p's factor is applied inside the fused inner CG only
   h.FS.build(pattern_of(Mnew), sS, o.ilu_k_S, 8);
@@ -1117,7 +1118,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.x.alloc(8 * h.ld);
   h.r.alloc(8 * h.ld);
   h.V.alloc(8 * h.ld * (m + 1));
-  h.Z.alloc((o.precision == FS_PREC_MIXED ? 4 : 8) * h.ld * m);
+  h.Z.alloc(8 * h.ld * m);  // FP64 or FP32 preconditioned vectors (the precision can be switched)
   h.ytmp.alloc(8 * h.ld);
   h.tp.alloc(8 * h.ld);
   h.cg.alloc(8 * 4 * ((h.np + 63) & ~int64_t(63)));
@@ -1347,6 +1348,15 @@ int fs_stokes_set_method(fs_handle* h, int32_t method) {
     return FS_ERR_BAD_INPUT;
   }
   h->o.method = method;
+  return FS_OK;
+}
+
+int fs_stokes_set_precision(fs_handle* h, int32_t precision) {
+  if (!h || precision < FS_PREC_FP64 || precision > FS_PREC_MIXED_FP64_VECTORS) {
+    fs::set_error("fs_stokes_set_precision: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  h->o.precision = precision;
   return FS_OK;
 }
 

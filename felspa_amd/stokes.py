@@ -236,6 +236,11 @@ class StokesLinearSystem:
         L.check(self._lib.fs_stokes_set_guess_mode(self.h, int(guess)), "fs_stokes_set_guess_mode")
         self.opts.guess = guess
 
+    def set_precision(self, precision):
+        """FS_PREC_* for the next solves on the same factorisation."""
+        L.check(self._lib.fs_stokes_set_precision(self.h, int(precision)), "fs_stokes_set_precision")
+        self.opts.precision = precision
+
     def vmult(self, x, precision=L.FS_PREC_FP64):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n)

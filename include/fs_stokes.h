@@ -165,6 +165,13 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const 
  * difference, done by the caller. */
 int fs_stokes_set_method(fs_handle* h, int32_t method);
 
+/* Switch the precision (fs_opts.precision, FS_PREC_*) of an existing handle
+ * between solves: every handle holds FP64 and FP32 copies of its matrices and
+ * factors, so one symbolic phase and one factor serve the plain FP64 and the
+ * mixed solve (Control::precision has no counterpart in the reference, whose
+ * solve is FP64 only). Reference-mode GMRES and SCR stay FP64-only. */
+int fs_stokes_set_precision(fs_handle* h, int32_t precision);
+
 /* Switch the guess mode (fs_opts.guess) of an existing handle between solves,
  * e.g. to compare FS_GUESS_LEARNED with FS_GUESS_ZERO on the same matrices. */
 int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess);
