@@ -1573,38 +1573,42 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ a, i
   return lo + __popcll(__ballot(lt));
 }
 
-// LDS of one stage-update workgroup: the row suffix's working values (and a
-// dump slot), a batch of 64 L columns' U spans and pivots, the suffix's
-// columns and the hash map
+// LDS of one stage-update workgroup: the G rows' working values from the
+// first updated column on (and a dump slot each), a batch of 64 L columns'
+// U spans and pivots, the columns and the hash map
 constexpr int UPD_BATCH = 64;
-__host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs) {
-  return (size_t)(mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 4) + (size_t)mcap * 4 + (size_t)hs * 4;
+__host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) {
+  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 4) + (size_t)mcap * 4 + (size_t)hs * 4;
 }
 
-// One stage update: rows[0 .. nrows) (ascending, each of at most mcap
-// entries, all after the stage) apply their external L columns in
-// [col_a, col_b), the rows of the finished stage. One workgroup of NT
-// threads per row, workgroups XCD-contiguous so that neighbouring rows, which
-// name the same finished rows, share an L2. The row from its first such
-// column on goes to LDS (values, columns, hash map); then for the L columns
-// in order, in batches of 64 whose U spans and pivots are read first: m =
-// w_t / u_kk goes to the factor, and the U part of row k (loaded three
-// columns ahead, M entries per thread per round) updates w_j -= m u_kj where
-// row i stores j. Then the rest of the suffix goes back to the factor.
-template <int NT, int M>
+// One stage update: the row groups starting at rows[0 .. nrows) (G
+// consecutive rows of one block with the same columns, e.g. the velocity
+// components of one node; ascending, each row of at most mcap entries, all
+// after the stage) apply their external L columns in [col_a, col_b), the
+// rows of the finished stage. One workgroup of NT threads per group,
+// workgroups XCD-contiguous so that neighbouring rows, which name the same
+// finished rows, share an L2. The rows from their first such column on go to
+// LDS (values per row, the shared columns and hash map); then for the L
+// columns in order, in batches of 64 whose U spans and pivots are read first:
+// each row's m = w_t / u_kk goes to the factor, and the U part of row k (a
+// stream of chunks of NT M entries, loaded three chunks ahead, each entry
+// looked up once for the G rows) updates w_j -= m u_kj where the rows store
+// j. Then the rest goes back to the factor.
+template <int NT, int M, int G>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
                                                            int nrows, int32_t col_a, int32_t col_b, int mcap,
                                                            int hs) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
-  double* w = (double*)smem;                  // [mcap + 1]; w[mcap]: dump slot
-  int64_t* mq = (int64_t*)(w + mcap + 1);     // [UPD_BATCH] first U entry of row k
-  double* mu = (double*)(mq + UPD_BATCH);     // [UPD_BATCH] pivot u_kk
-  int32_t* mn = (int32_t*)(mu + UPD_BATCH);   // [UPD_BATCH] U entries of row k
-  int32_t* lc = mn + UPD_BATCH;               // [mcap] columns of the suffix
-  uint32_t* tab = (uint32_t*)(lc + mcap);     // [hs] hash map
+  double* w = (double*)smem;                     // [G][mcap + 1]; w[g][mcap]: dump slot
+  int64_t* mq = (int64_t*)(w + G * (mcap + 1));  // [UPD_BATCH] first U entry of row k
+  double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] pivot u_kk
+  int32_t* mn = (int32_t*)(mu + UPD_BATCH);       // [UPD_BATCH] U entries of row k
+  int32_t* lc = mn + UPD_BATCH;                   // [mcap] columns
+  uint32_t* tab = (uint32_t*)(lc + mcap);         // [hs] hash map
   const uint32_t mask = (uint32_t)hs - 1;
   const int lg = __builtin_ctz((uint32_t)hs);
   const int tid = threadIdx.x, lane = tid & 63;
+  const int ld = mcap + 1;
   const int item = xcd_contiguous(blockIdx.x, gridDim.x);
   if (item >= nrows) return;
   const int32_t i = rows[item];
@@ -1617,12 +1621,16 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
   if (pa == pb) return;  // the whole workgroup: no barrier is passed
   const int m = len - pa;  // positions [pa, len): every column a U part of stage rows can name
   const int nc = pb - pa;
+  int64_t bg[G];  // the group's rows: row i + g starts at bg[g] + pa
+#pragma unroll
+  for (int g = 0; g < G; ++g) bg[g] = F.rp[i + g] + pa;
   for (int h = tid; h < hs; h += NT) tab[h] = NUM_EMPTY;
   lds_barrier();
   for (int u = tid; u < m; u += NT) {
     const int32_t c = F.ci[base + pa + u];
     lc[u] = c;
-    w[u] = F.v[base + pa + u];
+#pragma unroll
+    for (int g = 0; g < G; ++g) w[g * ld + u] = F.v[bg[g] + u];
     const uint32_t hh = num_hash(c);
     const uint32_t ent = ((hh & 0xFFFFu) << 16) | (uint32_t)u;
     for (uint32_t h = hh >> (32 - lg);; h = (h + 1) & mask)
@@ -1642,22 +1650,49 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
       mu[tid] = F.v[kb + kd];
     }
     lds_barrier();
-    // (every load is issued, from entry 0 when out of range, so every path
-    // has the same loads in flight)
-    auto fetch = [&](int c, int32_t* j, double* v) {
-      const bool ok = c < nb;
-      const int s = ok ? c : 0;
+    // The batch's U parts as one stream of chunks of NT M entries, chunk
+    // (c, off) = entries [off, off + NT M) of column c's U part (a column
+    // with an empty U part still has its chunk: its multipliers). Loads run
+    // three chunks ahead across column boundaries; a barrier precedes the
+    // first chunk of every column (its pivot entries must have all earlier
+    // columns' updates). Positions past the batch load from entry 0.
+    constexpr int CHUNK = NT * M;
+    auto adv = [&](int2 p) -> int2 {
+      if (p.x >= nb) return p;
+      p.y += CHUNK;
+      if (p.y >= mn[p.x]) {
+        p.x += 1;
+        p.y = 0;
+      }
+      return p;
+    };
+    auto issue = [&](int2 p, int32_t* j, double* v) {
+      const bool ok = p.x < nb;
+      const int s = ok ? p.x : 0;
       const int64_t q = mq[s];
       const int n = ok ? mn[s] : 0;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        const int idx = tid + r * NT;
+        const int idx = p.y + tid + r * NT;
         const int64_t a = idx < n ? q + idx : 0;
         j[r] = F.ci[a];
         v[r] = F.v[a];
       }
     };
-    auto update = [&](double mult, int n, int off, const int32_t* j, const double* v) {
+    double mult[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) mult[g] = 0.0;
+    auto process = [&](int2 p, const int32_t* j, const double* v) {
+      if (p.y == 0) {  // a new column: every earlier column's updates are in
+        lds_barrier();
+        const double piv = mu[p.x];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          mult[g] = w[g * ld + b0 + p.x] / piv;
+          if (tid == g) F.v[bg[g] + b0 + p.x] = mult[g];
+        }
+      }
+      const int n = mn[p.x];
       uint32_t h[M], e[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
@@ -1667,63 +1702,45 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
       int pos[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        const int p = num_probe(tab, lc, mask, j[r], num_hash(j[r]) & 0xFFFFu, h[r], e[r]);
-        pos[r] = off + tid + r * NT < n && p >= 0 ? p : mcap;
+        const int pp = num_probe(tab, lc, mask, j[r], num_hash(j[r]) & 0xFFFFu, h[r], e[r]);
+        pos[r] = p.y + tid + r * NT < n && pp >= 0 ? pp : mcap;
       }
-      double o[M];
 #pragma unroll
-      for (int r = 0; r < M; ++r) o[r] = w[pos[r]];
+      for (int g = 0; g < G; ++g) {
+        double o[M];
 #pragma unroll
-      for (int r = 0; r < M; ++r) w[pos[r]] = o[r] - mult * v[r];
-    };
-    auto apply = [&](int c, const int32_t* j, const double* v) {
-      // w at column b0 + c is final: earlier columns are applied, later
-      // ones only update positions to its right
-      const double mult = w[b0 + c] / mu[c];
-      if (tid == 0) F.v[base + pa + b0 + c] = mult;
-      const int n = mn[c];
-      update(mult, n, 0, j, v);
-      if (n > M * NT) {  // U parts longer than M NT: further rounds
-        const int64_t q = mq[c];
-        for (int off = M * NT; off < n; off += M * NT) {
-          int32_t jt[M];
-          double vt[M];
+        for (int r = 0; r < M; ++r) o[r] = w[g * ld + pos[r]];
 #pragma unroll
-          for (int r = 0; r < M; ++r) {
-            const int idx = off + tid + r * NT;
-            const int64_t a = idx < n ? q + idx : 0;
-            jt[r] = F.ci[a];
-            vt[r] = F.v[a];
-          }
-          update(mult, n, off, jt, vt);
-        }
+        for (int r = 0; r < M; ++r) w[g * ld + pos[r]] = o[r] - mult[g] * v[r];
       }
     };
-    fetch(0, ja, va);
-    fetch(1, jb, vb);
-    fetch(2, jc, vc);
-    int c = 0;
-    for (; c + 2 < nb; c += 3) {
-      apply(c, ja, va);
-      fetch(c + 3, ja, va);
-      lds_barrier();
-      apply(c + 1, jb, vb);
-      fetch(c + 4, jb, vb);
-      lds_barrier();
-      apply(c + 2, jc, vc);
-      fetch(c + 5, jc, vc);
-      lds_barrier();
+    int2 p0 = make_int2(0, 0);
+    int2 p1 = adv(p0), p2 = adv(p1);
+    issue(p0, ja, va);
+    issue(p1, jb, vb);
+    issue(p2, jc, vc);
+    for (;;) {
+      if (p0.x >= nb) break;
+      process(p0, ja, va);
+      const int2 p3 = adv(p2);
+      issue(p3, ja, va);
+      if (p1.x >= nb) break;
+      process(p1, jb, vb);
+      const int2 p4 = adv(p3);
+      issue(p4, jb, vb);
+      if (p2.x >= nb) break;
+      process(p2, jc, vc);
+      const int2 p5 = adv(p4);
+      issue(p5, jc, vc);
+      p0 = p3;
+      p1 = p4;
+      p2 = p5;
     }
-    if (c < nb) {
-      apply(c, ja, va);
-      lds_barrier();
-    }
-    if (c + 1 < nb) {
-      apply(c + 1, jb, vb);
-      lds_barrier();
-    }
+    lds_barrier();  // the batch's last updates, before its metadata is replaced
   }
-  for (int u = nc + tid; u < m; u += NT) F.v[base + pa + u] = w[u];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+    for (int u = nc + tid; u < m; u += NT) F.v[bg[g] + u] = w[g * ld + u];
 }
 
 // ---------------------------------------------------------------------------

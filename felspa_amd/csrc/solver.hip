@@ -237,36 +237,64 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   staged = max_len > 512 && sched.n_stages() > 1;
   if (staged) {
     const size_t lim = dense_wg_lds_limit();
+    // groups: up to 3 consecutive rows of one block with the same columns
+    // (the velocity components of a node), which share every lookup
+    std::vector<uint8_t> same(n, 0);
+    std::vector<int32_t> blk_of(n);
+    for (int b2 = 0; b2 < sched.n_blocks(); ++b2)
+      for (int32_t r = sched.blk_ptr[b2]; r < sched.blk_ptr[b2 + 1]; ++r) blk_of[r] = b2;
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t r = 1; r < n; ++r) {
+      const int64_t l0 = F.rp[r] - F.rp[r - 1], l1 = F.rp[r + 1] - F.rp[r];
+      same[r] = blk_of[r] == blk_of[r - 1] && l0 == l1 &&
+                std::equal(F.ci.begin() + F.rp[r], F.ci.begin() + F.rp[r + 1], F.ci.begin() + F.rp[r - 1]);
+    }
     static const int caps[] = {256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096, 5120, 6144, 8192};
-    std::vector<std::vector<int32_t>> lists(sizeof(caps) / sizeof(caps[0]));
-    for (int64_t r = 0; r < n; ++r) {
-      if (ol[r] == 0) continue;
-      const int len = (int)(F.rp[r + 1] - F.rp[r]);
-      size_t b = 0;
-      while (b + 1 < lists.size() && caps[b] < len) ++b;
-      if (caps[b] < len) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
-      lists[b].push_back((int32_t)r);
-    }
-    for (size_t b = 0; b < lists.size(); ++b) {
-      if (lists[b].empty()) continue;
-      NumBin nb;
-      nb.mcap = caps[b];
-      // hash load factor <= 1/4 unless that costs waves per CU, never above 1/2
-      int hmin = 64, hgood = 64;
-      while (hmin < 2 * nb.mcap) hmin *= 2;
-      while (hgood < 4 * nb.mcap) hgood *= 2;
-      const size_t wmin = ilu_update_lds_bytes(nb.mcap, hmin), wgood = ilu_update_lds_bytes(nb.mcap, hgood);
-      if (wmin > lim) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
-      nb.hs = (wgood <= lim && lim / wgood >= lim / wmin) ? hgood : hmin;
-      nb.nrows = (int)lists[b].size();
-      nb.rows.upload(lists[b].data(), 4 * lists[b].size());
-      nb.first.resize(sched.n_stages());
-      for (int st = 0; st < sched.n_stages(); ++st) {
-        const int32_t after = sched.blk_ptr[sched.stage_ptr[st + 1]];
-        nb.first[st] = (int)(std::lower_bound(lists[b].begin(), lists[b].end(), after) - lists[b].begin());
+    constexpr int NC = sizeof(caps) / sizeof(caps[0]);
+    std::vector<std::vector<int32_t>> lists(NC * 3);  // [cap][G - 1]
+    for (int64_t r = 0; r < n;) {
+      int g = 1;
+      while (g < 3 && r + g < n && same[r + g]) ++g;
+      if (ol[r] > 0) {
+        const int len = (int)(F.rp[r + 1] - F.rp[r]);
+        int c = 0;
+        while (c + 1 < NC && caps[c] < len) ++c;
+        if (caps[c] < len) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
+        lists[3 * c + g - 1].push_back((int32_t)r);
       }
-      bins.push_back(std::move(nb));
+      r += g;
     }
+    for (int c = 0; c < NC; ++c)
+      for (int g = 3; g >= 1; --g) {
+        auto& lst = lists[3 * c + g - 1];
+        if (lst.empty()) continue;
+        NumBin nb;
+        nb.mcap = caps[c];
+        nb.G = g;
+        // hash load factor <= 1/4 unless that costs workgroups per CU,
+        // never above 1/2; groups that do not fit split into single rows
+        int hmin = 64, hgood = 64;
+        while (hmin < 2 * nb.mcap) hmin *= 2;
+        while (hgood < 4 * nb.mcap) hgood *= 2;
+        const size_t wmin = ilu_update_lds_bytes(nb.mcap, hmin, g), wgood = ilu_update_lds_bytes(nb.mcap, hgood, g);
+        if (wmin > lim) {
+          if (g == 1) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
+          auto& one = lists[3 * c];
+          for (int32_t r0 : lst)
+            for (int k2 = 0; k2 < g; ++k2) one.push_back(r0 + k2);
+          std::sort(one.begin(), one.end());
+          continue;
+        }
+        nb.hs = (wgood <= lim && lim / wgood >= lim / wmin) ? hgood : hmin;
+        nb.nrows = (int)lst.size();
+        nb.rows.upload(lst.data(), 4 * lst.size());
+        nb.first.resize(sched.n_stages());
+        for (int st2 = 0; st2 < sched.n_stages(); ++st2) {
+          const int32_t after = sched.blk_ptr[sched.stage_ptr[st2 + 1]];
+          nb.first[st2] = (int)(std::lower_bound(lst.begin(), lst.end(), after) - lst.begin());
+        }
+        bins.push_back(std::move(nb));
+      }
   }
   for (int64_t i = 0; i < n; ++i) {
     lc[i + 1] += lc[i];
@@ -551,6 +579,13 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     need_v32 = !(fits(8, 8, dense_rc<double, double>()) && fits(4, 8, dense_rc<float, double>()) &&
                  fits(4, 4, dense_rc<float, float>()));
   }
+  // every array the numeric and stage kernels dereference is on the device
+  // (a launch on an unset one would fault the device, not report an error)
+  if (!rp.p || !ci.p || !v64.p || !o_lint.p || !o_diag.p || !o_uext.p || !blk_ptr.p || !stage_ptr.p || !lcp.p ||
+      !ucp.p || !lrow.p || !urow.p || !lpos.p || !upos.p || !lv64.p || !uv64.p ||
+      (dense && (!inv_off.p || !linv64.p || !uinv64.p || !xldesc.p || !xudesc.p || !xlrng.p || !xurng.p ||
+                 !xlv64.p || !xuv64.p || !xlv32.p || !xuv32.p || !dpart.p)))
+    throw FsError(FS_ERR_HIP, "ILU factor: device arrays missing after the symbolic phase");
   if (keep_pattern) host_pattern = std::move(F);
 }
 
@@ -623,18 +658,21 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
         if (cnt_rows <= 0) continue;
         // one wave per row where several rows share a CU's LDS, four where
         // a row's LDS leaves room for one or two
-        const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs);
+        const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs, nb.G);
         const int32_t* rl = nb.rows.as<int32_t>() + nb.first[s];
-#define FS_UPD(NT, M)                                                                                          \
+#define FS_UPD(NT, M, G)                                                                                       \
   {                                                                                                            \
-    auto fn = &k_ilu_numeric_update<NT, M>;                                                                    \
+    auto fn = &k_ilu_numeric_update<NT, M, G>;                                                                 \
     if (lds > 64 * 1024)                                                                                       \
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
                 "hipFuncSetAttribute");                                                                        \
     hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, cnt_rows, ca, cb, nb.mcap, nb.hs);        \
   }
-        if (nb.mcap <= 1024) FS_UPD(64, 8)
-        else FS_UPD(256, 4)
+        if (nb.mcap <= 1024) {
+          if (nb.G == 3) FS_UPD(64, 8, 3) else if (nb.G == 2) FS_UPD(64, 8, 2) else FS_UPD(64, 8, 1)
+        } else {
+          if (nb.G == 3) FS_UPD(256, 8, 3) else if (nb.G == 2) FS_UPD(256, 8, 2) else FS_UPD(256, 8, 1)
+        }
 #undef FS_UPD
         cnt.launches++;
       }

@@ -103,7 +103,7 @@ struct IluDev {
   // parts): rows with external L columns in bins of length, each bin's rows
   // ascending; first[s] = first row of the bin after stage s
   struct NumBin {
-    int mcap = 0, hs = 0;
+    int mcap = 0, hs = 0, G = 1;  // row cap, hash size, rows per group
     DevBuf rows;
     std::vector<int> first;
     int nrows = 0;
