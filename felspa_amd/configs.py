@@ -17,13 +17,13 @@ CONFIGS = {
                eta_matrix=1.0, eta_incl=100.0, rho_matrix=1.0, rho_incl=1.1, amp=0.02,
                gravity=1.0, ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
     "C3": dict(dim=3, n_cells=32, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_LINEAR_SHEAR,
-               eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+               eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=3),
     "C4": dict(dim=2, n_cells=128, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 8), axes=(0.05, 0.2), logc=(0.0, 3.0),
                ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
     "C5": dict(dim=3, n_cells=48, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 4), axes=(0.08, 0.25), logc=(1.0, 3.0),
-               ilu_k_A=2, ilu_k_S=0, hint_nodes=7),
+               ilu_k_A=2, ilu_k_S=0, hint_nodes=3),
 }
 
 _SOLVER_KEYS = ("ilu_k_A", "ilu_k_S", "hint_nodes")

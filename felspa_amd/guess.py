@@ -54,10 +54,14 @@ class GuessModel:
     def load(cls, path):
         return cls(path=path)
 
-    def __del__(self):
+    def close(self):
+        """Release the model's device memory now (also done on garbage collection)."""
         if getattr(self, "h", None):
             self._lib.fs_guess_model_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def save(self, path):
         d = os.path.dirname(os.path.abspath(path))
@@ -193,28 +197,38 @@ def lattice_to_dofs(S, T):
 # ---------------------------------------------------------------------------
 # data generation and training
 def generate_solutions(cfg_name, samples, n_cells=None, restart=30, inner_a=60, rtol=1e-8, log=None,
-                       budget_s=0.0):
+                       budget_s=0.0, ilu_k=None, prefetch=3):
     """Solve the seeded problems of a configuration with the GPU solver
-    (mixed-precision FGMRES to `rtol` from a zero guess), stopping early once
-    `budget_s` seconds have passed (0: no limit); returns
-    log10(eta) on the pressure lattice [S, (N+1)^d] and the node-lattice
-    solutions [S, d+1, (2N+1)^d] (float32), plus per-solve statistics."""
+    (mixed-precision FGMRES to `rtol` from a zero guess; ILU level `ilu_k`,
+    default the configuration's: the solution does not depend on it),
+    stopping early once `budget_s` seconds have passed (0: no limit); the
+    next `prefetch` systems are assembled on host threads while the GPU
+    solves. Returns log10(eta) on the pressure lattice [S, (N+1)^d] and the
+    node-lattice solutions [S, d+1, (2N+1)^d] (float32), plus per-solve
+    statistics."""
+    from concurrent.futures import ThreadPoolExecutor
     from .configs import make_problem, solver_settings
     from .stokes import StokesLinearSystem
     from .system import StokesSystem
     st_cfg = solver_settings(cfg_name)
+    kA = st_cfg["ilu_k_A"] if ilu_k is None else ilu_k
     lin = S0 = None
     logeta, targets, stats = [], [], []
     t0 = time.time()
+    pool = ThreadPoolExecutor(max_workers=max(1, prefetch))
+    make = lambda s: StokesSystem(make_problem(cfg_name, n_cells=n_cells, sample=s))
+    pending = [pool.submit(make, s) for s in samples[:prefetch]]
     for i, s in enumerate(samples):
         if budget_s and time.time() - t0 > budget_s:
             if log:
                 log("data budget reached after %d of %d solves" % (i, len(samples)))
             break
-        S = StokesSystem(make_problem(cfg_name, n_cells=n_cells, sample=s))
+        S = pending.pop(0).result()
+        if i + prefetch < len(samples):
+            pending.append(pool.submit(make, samples[i + prefetch]))
         if lin is None:
             S0 = S
-            lin = StokesLinearSystem(S.blocks, A_lof=st_cfg["ilu_k_A"], S_lof=st_cfg["ilu_k_S"],
+            lin = StokesLinearSystem(S.blocks, A_lof=kA, S_lof=st_cfg["ilu_k_S"],
                                      precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
                                      block_hint=S.block_hint(st_cfg["hint_nodes"]),
                                      constraints=S.constrained, restart=restart, inner_a_iters=inner_a,
@@ -232,9 +246,15 @@ def generate_solutions(cfg_name, samples, n_cells=None, restart=30, inner_a=60, 
         targets.append(lattice_targets(S, x))
         stats.append(dict(sample=s, outer=st["outer_iters"], res=st["true_rel_residual"],
                           ms=st["t_total_ms"]))
-        if log and (i % 32 == 0 or i == len(samples) - 1):
+        if S is not S0:
+            del S
+        if log and (i % 16 == 0 or i == len(samples) - 1):
             log("solved %d/%d (sample %d: %d outer, %.1f ms) %.1fs" % (i + 1, len(samples), s, st["outer_iters"],
                                                                        st["t_total_ms"], time.time() - t0))
+    for f in pending:
+        f.cancel()
+    pool.shutdown(wait=True)
+    lin.close()
     return np.stack(logeta), np.stack(targets), stats
 
 
@@ -252,6 +272,7 @@ def train(model, d_X, d_T, n_cells, epochs=100, batch=16, lr=2e-3, lr_final=1e-4
     hist = []
     step = 0
     t0 = time.time()
+    t_log = t0
     for ep in range(epochs):
         perm = torch.randperm(S, generator=g).to(d_X.device)
         tot = 0.0
@@ -268,7 +289,8 @@ def train(model, d_X, d_T, n_cells, epochs=100, batch=16, lr=2e-3, lr_final=1e-4
             vl = evaluate_loss(model, d_Xv, d_Tv, n_cells, batch)
         hist.append((tot / nb, vl))
         over = budget_s and time.time() - t0 > budget_s
-        if log and (ep % 10 == 0 or ep == epochs - 1 or over):
+        if log and (ep % 10 == 0 or ep == epochs - 1 or over or time.time() - t_log > 60):
+            t_log = time.time()
             log("epoch %d: train %.4e held-out %s (%.1fs)" % (ep, tot / nb, "%.4e" % vl if vl is not None else "-",
                                                               time.time() - t0))
         if over:

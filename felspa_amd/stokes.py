@@ -119,10 +119,15 @@ class StokesLinearSystem:
                 "fs_stokes_create")
         self.t_create_s = time.perf_counter() - t0
 
-    def __del__(self):
+    def close(self):
+        """Release the handle's device memory now (also done on garbage
+        collection); the object is unusable afterwards."""
         if getattr(self, "h", None):
             self._lib.fs_stokes_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
     def refactor(self, blocks):
         """New values on the same sparsity (use_previous_sparsity = true)."""

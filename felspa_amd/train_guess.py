@@ -4,6 +4,10 @@
                                      [--epochs 150] [--out felspa_amd/weights/guess_ellipses_2d.fsgm]
                                      [--summary summary.json]
 
+    3D (48^3 ellipsoid family, trained at 32^3):
+    python -m felspa_amd.train_guess --config C5 --n-cells 32 --train 256 --held-out 16 --eval 4 \
+        --modes 8 --ilu-k 1 --inner-a 15 --epochs 120 --summary felspa_amd/weights/guess_ellipsoids_3d.json
+
 1. Generates the data with the GPU solver itself: samples 0..train-1 (and
    train..train+held_out-1 held out) of the configuration, each solved by
    mixed-precision FGMRES to rtol 1e-8 from a zero guess (the method without
@@ -48,6 +52,10 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--data-rtol", type=float, default=1e-6,
                     help="rtol of the training solves (targets need far less than the solve's 1e-8)")
+    ap.add_argument("--ilu-k", type=int, default=None,
+                    help="ILU level of the data solves and of the held-out evaluation (default: the configuration's)")
+    ap.add_argument("--inner-a", type=int, default=60, help="inner ILU-PCG steps on A per outer iteration")
+    ap.add_argument("--restart", type=int, default=60, help="FGMRES restart of the data and evaluation solves")
     ap.add_argument("--data-budget-s", type=float, default=0.0, help="stop generating after this long (0: none)")
     ap.add_argument("--train-budget-s", type=float, default=0.0, help="stop training after this long (0: none)")
     ap.add_argument("--out", default=None)
@@ -68,7 +76,8 @@ def main(argv=None):
     held = list(range(args.train, args.train + args.held_out))
     order = held + list(range(args.train))
     le, T, stats = G.generate_solutions(args.config, order, n_cells=nc, rtol=args.data_rtol, log=log,
-                                        budget_s=args.data_budget_s)
+                                        budget_s=args.data_budget_s, ilu_k=args.ilu_k, restart=args.restart,
+                                        inner_a=args.inner_a)
     t_data = time.time() - t0
     log("data: %d solves in %.1fs, mean outer %.1f" % (len(stats), t_data, np.mean([s["outer"] for s in stats])))
     nh = args.held_out
@@ -105,10 +114,10 @@ def main(argv=None):
     for s in samples[ntr:ntr + args.eval]:
         S = StokesSystem(fa.make_problem(args.config, n_cells=nc, sample=s))
         if lin is None:
-            lin = fa.StokesLinearSystem(S.blocks, A_lof=st_cfg["ilu_k_A"], S_lof=st_cfg["ilu_k_S"],
-                                        precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
+            lin = fa.StokesLinearSystem(S.blocks, A_lof=st_cfg["ilu_k_A"] if args.ilu_k is None else args.ilu_k,
+                                        S_lof=st_cfg["ilu_k_S"], precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
                                         block_hint=S.block_hint(st_cfg["hint_nodes"]), constraints=S.constrained,
-                                        restart=60, inner_a_iters=60)
+                                        restart=args.restart, inner_a_iters=args.inner_a)
         else:
             lin.refactor(S.blocks)
         lin.set_guess_field(model, S)
@@ -124,8 +133,10 @@ def main(argv=None):
                        res_zero=z["true_rel_residual"], res_learned=g["true_rel_residual"]))
         log("held-out %d: outer zero %d learned %d, ms %.1f -> %.1f, guess residual %.3e" % (
             s, z["outer_iters"], g["outer_iters"], z["t_total_ms"], g["t_total_ms"], g["guess_rel_residual"]))
+    if lin is not None:
+        lin.close()
     summary = dict(config=args.config, n_cells=nc, train=ntr, held_out=args.held_out, epochs=len(hist),
-                   data_rtol=args.data_rtol, mean_outer_data=float(np.mean([s["outer"] for s in stats])),
+                   data_rtol=args.data_rtol, ilu_k=args.ilu_k, inner_a=args.inner_a, restart=args.restart, mean_outer_data=float(np.mean([s["outer"] for s in stats])),
                    net=model.cfg, n_params=model.n_params, t_data_s=t_data, t_train_s=t_train,
                    final_train_loss=hist[-1][0], final_held_out_loss=hist[-1][1],
                    mean_outer_zero=float(np.mean([e["outer_zero"] for e in ev])),
