@@ -455,11 +455,18 @@ struct fs_handle {
     cnt.bytes += (double)n * (16.0 + dim * sizeof(TZ));
   }
 
+  // Krylov basis size: fs_opts.restart, or when 0 the method's own (deal.II
+  // SolverGMRES max_n_tmp_vectors 25 -> 23 for the reference method,
+  // stokes_ilu.cc:538; 60 for the flexible method)
+  int restart_of(int method) const {
+    return o.restart > 0 ? o.restart : (method == FS_METHOD_GMRES_REF ? 23 : 60);
+  }
+
   // right-preconditioned flexible GMRES, FP64 true residual at each restart;
   // the preconditioner streams TV matrix / factor values and computes in T
   template <typename TV, typename T>
   void fgmres(double tol_abs, int max_steps, fs_stats& s) {
-    const int m = o.restart;
+    const int m = restart_of(FS_METHOD_FGMRES);
     double* Vb = dptr<double>(V);
     T* Zb = (T*)Z.p;
     int acc = 0, cycles = 0;
@@ -504,7 +511,7 @@ struct fs_handle {
 
   // deal.II SolverGMRES, left preconditioning (stokes_ilu.cc:536-542)
   void gmres_ref(double bnorm, int max_steps, fs_stats& s) {
-    const int m = o.restart;
+    const int m = restart_of(FS_METHOD_GMRES_REF);
     double* Vb = dptr<double>(V);
     double* p = dptr<double>(r);
     Ctrl c;
@@ -1065,7 +1072,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   if (opts) h.o = *opts;
   else fs_opts_default(&h.o);
   auto& o = h.o;
-  if (o.restart < 1 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [1, 63]");
+  if (o.restart < 0 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [0, 63]");
   if (o.precision < FS_PREC_FP64 || o.precision > FS_PREC_MIXED_FP64_VECTORS)
     throw FsError(FS_ERR_BAD_INPUT, "unknown precision");
   if (o.ilu_k_A < 0 || o.ilu_k_S < 0 || o.ilu_k_A > 20 || o.ilu_k_S > 20)
@@ -1113,7 +1120,8 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.hMp.assign(h.Mp.nnz, 0.0);
   // device vectors
   h.ld = (h.n + 63) & ~int64_t(63);
-  const int m = o.restart;
+  // (the method can be switched later: room for either basis)
+  const int m = std::max(h.restart_of(FS_METHOD_GMRES_REF), h.restart_of(FS_METHOD_FGMRES));
   h.b.alloc(8 * h.ld);
   h.x.alloc(8 * h.ld);
   h.r.alloc(8 * h.ld);
@@ -1463,7 +1471,7 @@ void fs_opts_default(fs_opts* o) {
   o->ordering = FS_ORDER_BLOCK_COLOR;
   o->ilu_k_A = 1;
   o->ilu_k_S = 0;
-  o->restart = 23;
+  o->restart = 0;  // the method's own: 23 (reference GMRES), 60 (FGMRES)
   o->max_outer = 0;
   o->rtol = 1e-8;
   o->reduce = 1e-6;
@@ -1476,7 +1484,7 @@ void fs_opts_default(fs_opts* o) {
   o->block_rows = 0;
   o->block_hint = nullptr;
   o->verbose = 0;
-  o->inner_a_iters = 30;
+  o->inner_a_iters = 60;
 }
 
 const char* fs_last_error(const fs_handle*) { return fs::get_error(); }

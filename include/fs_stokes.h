@@ -73,7 +73,9 @@ typedef struct fs_opts {
   int32_t ordering;      /* FS_ORDER_* */
   int32_t ilu_k_A;       /* Control::set_level_of_fill A (stokes_ilu.cc:139-147), default 1 */
   int32_t ilu_k_S;       /* ... S, default 0 */
-  int32_t restart;       /* Krylov basis size; 23 = deal.II max_n_tmp_vectors 25 - 2 */
+  int32_t restart;       /* Krylov basis size; 0 (default): the method's own, 23 for
+                            FS_METHOD_GMRES_REF (deal.II max_n_tmp_vectors 25 - 2,
+                            stokes_ilu.cc:538) and 60 for FS_METHOD_FGMRES */
   int32_t max_outer;     /* 0: n_u + n_p (stokes_ilu.cc:534) */
   double rtol;           /* tol = rtol ||rhs|| (stokes_ilu.cc:533), default 1e-8 */
   double reduce;         /* ReductionControl reduce (stokes_common.h:343-348), default 1e-6 */
@@ -87,9 +89,10 @@ typedef struct fs_opts {
   int32_t block_rows;    /* target rows per block for FS_ORDER_BLOCK_COLOR (0: default) */
   const int32_t* block_hint; /* optional [n_u] block id per velocity row (e.g. geometric boxes) */
   int32_t verbose;
-  int32_t inner_a_iters; /* A^{-1} in the block preconditioner: 0 = one ILU(k)
-                            application (the reference's APreconditioner),
-                            >0 = that many ILU(k)-preconditioned CG steps on A */
+  int32_t inner_a_iters; /* A^{-1} in the block preconditioner of FS_METHOD_FGMRES:
+                            0 = one ILU(k) application (the reference's
+                            APreconditioner), >0 = that many ILU(k)-preconditioned
+                            CG steps on A (default 60) */
 } fs_opts;
 
 void fs_opts_default(fs_opts* opts);

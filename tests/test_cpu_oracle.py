@@ -226,3 +226,52 @@ def test_spmv_against_dense(oracle):
         y = oracle.spmv(M, xv)
         want = M.toarray() @ xv
         assert np.abs(y - want).max() <= 1e-13 * np.abs(M).sum(axis=1).max() * np.abs(xv).max()
+
+
+def test_scaling_coeffs_known_answer(oracle):
+    """StokesScalingOperator::initialize (stokes_common.implement.h:13-53) on a
+    hand-worked block pair, n_u = 3, n_p = 2:
+      A row maxima |a| = 4, 9, 16        -> s_u = sqrt(.) = 2, 3, 4
+      B^T row maxima |b| = 1, 3, 2       -> ghat = (1, 3, 2), |ghat|^2 = 14
+      P = sum 1/s_u^2 = 1/4 + 1/9 + 1/16 = 61/144
+      s_p = P * sqrt(P |ghat|^2) / n_p = (61/144) sqrt(854/144) / 2
+          = 61 sqrt(854) / 3456 = 0.5158043929214988
+    (dividing by n_u, dropping the square root, or summing |ghat| instead of
+    its squares each gives another number)."""
+    A = sp.csr_matrix(np.array([[4.0, -1.0, 0.0], [-1.0, 9.0, 2.0], [0.0, 2.0, -16.0]]))
+    Bt = sp.csr_matrix(np.array([[1.0, -0.5], [3.0, 0.0], [0.0, -2.0]]))
+    su, sp_ = oracle.scaling_coeffs(A, Bt, 2)
+    assert np.array_equal(su, [2.0, 3.0, 4.0])
+    assert sp_.shape == (2,)
+    assert abs(sp_[0] - 0.5158043929214988) <= 1e-15 and sp_[1] == sp_[0]
+
+
+def _pattern_rows(F):
+    return [list(F.indices[F.indptr[r]:F.indptr[r + 1]]) for r in range(F.shape[0])]
+
+
+def test_level_rule_first_insertion_known_answer(oracle):
+    """symbolic_factorize_generic (ilu.implement.h:245-435) keeps the level a
+    fill was first inserted with (the std::min at :343-348 only sees an
+    existing entry's level, and an existing entry is never re-inserted), where
+    the textbook rule (Saad, Alg. 10.5) lowers it. Hand-worked, k = 2:
+      row 2 = {1, 2}: L column 1 brings row 1's (1,5), level 0+0+1 = 1;
+      row 7 = {2, 4, 7}: L column 2 (level 0) brings (2,5) at 1+0+1 = 2;
+      L column 4 (level 0) would give (4,5) at 0+0+1 = 1, but (7,5) exists
+      and keeps level 2 = k, so column 5 is not expanded: no (7,6).
+    The textbook rule lowers (7,5) to 1 and expands it, adding (7,6) at
+    0+1+1 = 2; with k = 3 the reference expands level 2 too and adds (7,6)."""
+    rows = {0: [0], 1: [1, 5], 2: [1, 2], 3: [3], 4: [4, 5], 5: [5, 6], 6: [6], 7: [2, 4, 7]}
+    r = np.concatenate([[i] * len(c) for i, c in rows.items()])
+    c = np.concatenate(list(rows.values()))
+    A = sp.csr_matrix((np.full(len(r), 1.0) + 4.0 * (r == c), (r, c)), shape=(8, 8))
+    A.sort_indices()
+    want2 = [[0], [1, 5], [1, 2, 5], [3], [4, 5], [5, 6], [6], [2, 4, 5, 7]]
+    want3 = want2[:7] + [[2, 4, 5, 6, 7]]
+    import felspa_amd as fa
+    for k, want in ((2, want2), (3, want3)):
+        assert _pattern_rows(oracle.SparseILU(A, k).factors()) == want
+        assert _pattern_rows(fa.ilu_symbolic(A, k)) == want
+    # the textbook levels give the larger pattern at k = 2
+    tb = textbook_levels(A, 2)
+    assert tb[7, 6] and not _structure(oracle.SparseILU(A, 2).factors())[7, 6]

@@ -71,7 +71,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_default_options_follow_reference():
     o = fa.stokes.default_opts()
-    assert o.restart == 23           # max_n_tmp_vectors = 25 (stokes_ilu.cc:538)
+    # restart 0: each method's own basis, 23 for the reference GMRES
+    # (max_n_tmp_vectors = 25, stokes_ilu.cc:538), 60 for FGMRES, the default
+    # method, with 60 inner ILU-PCG steps on A (the converging configuration)
+    assert o.restart == 0 and o.method == L.FS_METHOD_FGMRES and o.inner_a_iters == 60
     assert o.rtol == 1e-8            # tol = 1e-8 ||rhs|| (stokes_ilu.cc:533)
     assert o.reduce == 1e-6          # StokesSolverControl reduce (stokes_common.h:343)
     assert o.inner_cg_tol == 1e-12   # InverseMatrixCG (linear_operators.h:44-45)
