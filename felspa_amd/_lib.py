@@ -71,7 +71,7 @@ class FsSystemInfo(C.Structure):
 EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
-    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
+    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_Tvmult", "fs_ilu_get_factors",
     "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
@@ -111,6 +111,8 @@ def lib():
         "fs_stokes_set_precision": (i32, [V, i32]),
         "fs_stokes_set_method": (i32, [V, i32]),
         "fs_stokes_get_ordering": (i32, [V, i32, P(i32), P(i32)]),
+        "fs_stokes_get_factor_info": (i32, [V, i32, P(i64), P(i64), P(i64), P(i32)]),
+        "fs_stokes_get_schedule": (i32, [V, i32, P(i32), P(i32), P(i32), P(i32)]),
         "fs_stokes_time_kernel": (i32, [V, i32, i32, i32, P(f64), P(f64)]),
         "fs_last_error": (C.c_char_p, [V]),
         "fs_stokes_destroy": (None, [V]),

@@ -210,6 +210,18 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       max_len = std::max<int>(max_len, (int)len);
     }
   }
+  // numeric work of the IKJ factorisation: sum over L entries (i, k) of the
+  // length of row k's U part (each such entry streams that U part once)
+  {
+    int64_t upd = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : upd)
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t p = F.rp[i]; p < F.rp[i] + od[i]; ++p) {
+        const int32_t kk = F.ci[p];
+        upd += F.rp[kk + 1] - F.rp[kk] - od[kk] - 1;
+      }
+    ikj_updates = upd;
+  }
   // per stage: the longest row and L part (the numeric kernel's LDS), and
   // the longest U part of any row up to the stage's end (the rows its L
   // columns can name: the numeric kernel's loads in flight)

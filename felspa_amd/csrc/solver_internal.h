@@ -109,6 +109,7 @@ struct IluDev {
     int nrows = 0;
   };
   bool staged = false;
+  int64_t ikj_updates = 0;  // numeric work: sum over L entries (i, k) of |U part of row k|
   bool need_v32 = true;      // FP32 CSR copy of the factor (substitution kernels)
   bool keep_pattern = true;  // keep the host pattern (factor queries, transposed factor)
   std::vector<NumBin> bins;

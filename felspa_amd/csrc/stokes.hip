@@ -1308,6 +1308,34 @@ int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* 
   return FS_OK;
 }
 
+int fs_stokes_get_schedule(fs_handle* h, int32_t which, int32_t* blk_ptr, int32_t* n_blocks, int32_t* stage_ptr,
+                           int32_t* n_stages) {
+  if (!h || (which != 0 && which != 1)) {
+    fs::set_error("fs_stokes_get_schedule: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  const Schedule& s = which == 0 ? h->FA.sched : h->FS.sched;
+  if (blk_ptr) std::copy(s.blk_ptr.begin(), s.blk_ptr.end(), blk_ptr);
+  if (stage_ptr) std::copy(s.stage_ptr.begin(), s.stage_ptr.end(), stage_ptr);
+  if (n_blocks) *n_blocks = s.n_blocks();
+  if (n_stages) *n_stages = s.n_stages();
+  return FS_OK;
+}
+
+int fs_stokes_get_factor_info(fs_handle* h, int32_t which, int64_t* n, int64_t* nnz, int64_t* ikj_updates,
+                              int32_t* n_stages) {
+  if (!h || (which != 0 && which != 1)) {
+    fs::set_error("fs_stokes_get_factor_info: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  const IluDev& F = which == 0 ? h->FA : h->FS;
+  if (n) *n = F.n;
+  if (nnz) *nnz = F.nnz;
+  if (ikj_updates) *ikj_updates = F.ikj_updates;
+  if (n_stages) *n_stages = F.sched.n_stages();
+  return FS_OK;
+}
+
 int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const float* log_eta, int32_t n_cells,
                               const int64_t* vnode, const int8_t* vcomp, const int64_t* pnode) {
   if (!h || !model || !log_eta || !vnode || !vcomp || !pnode || n_cells < 1) {

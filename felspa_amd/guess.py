@@ -20,6 +20,7 @@ from . import _lib as L
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS_DIR = os.path.join(HERE, "weights")
 DEFAULT_2D = os.path.join(WEIGHTS_DIR, "guess_ellipses_2d.fsgm")
+DEFAULT_3D = os.path.join(WEIGHTS_DIR, "guess_ellipsoids_3d.fsgm")
 
 
 def _ptr(t):

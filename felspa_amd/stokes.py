@@ -241,6 +241,24 @@ class StokesLinearSystem:
         L.check(self._lib.fs_stokes_set_guess_mode(self.h, int(guess)), "fs_stokes_set_guess_mode")
         self.opts.guess = guess
 
+    def factor_info(self, which=0):
+        """ILU factor of A (0) or M_p (1): dict n, nnz, ikj_updates, n_stages."""
+        n, nnz, upd, ns = L.i64(), L.i64(), L.i64(), L.i32()
+        L.check(self._lib.fs_stokes_get_factor_info(self.h, which, C.byref(n), C.byref(nnz), C.byref(upd),
+                                                    C.byref(ns)), "fs_stokes_get_factor_info")
+        return dict(n=n.value, nnz=nnz.value, ikj_updates=upd.value, n_stages=ns.value)
+
+    def schedule(self, which=0):
+        """(blk_ptr, stage_ptr) of the ILU schedule of A (0) or M_p (1)."""
+        nb, ns = L.i32(), L.i32()
+        L.check(self._lib.fs_stokes_get_schedule(self.h, which, None, C.byref(nb), None, C.byref(ns)),
+                "fs_stokes_get_schedule")
+        blk = np.zeros(nb.value + 1, np.int32)
+        stg = np.zeros(ns.value + 1, np.int32)
+        L.check(self._lib.fs_stokes_get_schedule(self.h, which, blk.ctypes.data_as(L.P(L.i32)), None,
+                                                 stg.ctypes.data_as(L.P(L.i32)), None), "fs_stokes_get_schedule")
+        return blk, stg
+
     def set_precision(self, precision):
         """FS_PREC_* for the next solves on the same factorisation."""
         L.check(self._lib.fs_stokes_set_precision(self.h, int(precision)), "fs_stokes_set_precision")

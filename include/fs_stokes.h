@@ -183,6 +183,20 @@ int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess);
  * M_p (which=1) blocks: perm[new] = old. */
 int fs_stokes_get_ordering(fs_handle* h, int32_t which, int32_t* perm, int32_t* n_stages);
 
+/* The block / stage schedule of that ordering (solver order): blk_ptr
+ * [n_blocks+1] row ranges of the blocks, stage_ptr [n_stages+1] block ranges
+ * of the colour stages. Pass NULL arrays to query the sizes. */
+int fs_stokes_get_schedule(fs_handle* h, int32_t which, int32_t* blk_ptr, int32_t* n_blocks, int32_t* stage_ptr,
+                           int32_t* n_stages);
+
+/* Size of the ILU(k) factor of A (which=0) or M_p (which=1): rows, stored
+ * entries (diagonal included), the numeric factorisation's work (the sum over
+ * L entries (i, k) of the length of row k's U part, which IKJ streams once
+ * per such entry) and the number of colour stages. Query only; no device
+ * work. */
+int fs_stokes_get_factor_info(fs_handle* h, int32_t which, int64_t* n, int64_t* nnz, int64_t* ikj_updates,
+                              int32_t* n_stages);
+
 /* Kernel timing for roofline reports: runs `reps` launches of one kernel
  * class on the handle's own stream between HIP events, with the current
  * factorisation and solver-order data, and returns the mean duration of one
