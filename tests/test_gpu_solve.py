@@ -186,19 +186,23 @@ def _refined_truth(S, steps=2):
     return _refined[id(S)]
 
 
-@pytest.mark.parametrize("cfg,n,kA,hint", [("C1", 64, 0, 5), ("C4", 128, 1, 5)])
+@pytest.mark.parametrize("cfg,n,kA,hint,p_bar", [("C1", 64, 0, 5, 1e-4), ("C4", 128, 1, 5, 2e-6)])
 @pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED])
-def test_full_size_parity_against_direct_solve(cfg, n, kA, hint, precision):
+def test_full_size_parity_against_direct_solve(cfg, n, kA, hint, p_bar, precision):
     """The product method (block-colour ILU, FGMRES with inner ILU-PCG on A;
     FP64, and FP32 preconditioner inside FP64 FGMRES) at the configurations'
     full sizes, C1 64^2 and C4 128^2, against a refined sparse direct solve:
-    velocity within the 1e-7 relative L2 of the accuracy target.
-    Both systems are ill-conditioned in the pressure: one refinement step
-    moves the direct solution's pressure by 3e-7 (C4) to 5e-7 (C1) relative,
-    and the oracle's FP64 CPU restatement of this method reaches 7e-7 on the
-    mean-free pressure at rtol 1e-14 (C4 sample 0); the mean-free pressure is
-    held to 2e-6 accordingly. rtol 1e-14 (at 1e-12 the same oracle is still
-    3.7e-6 from the direct solution in velocity on C4)."""
+    velocity within the 1e-7 relative L2 of the accuracy target, at rtol
+    1e-14 (at 1e-12 the oracle's FP64 CPU restatement of this method is still
+    3.7e-6 from the direct solution in velocity on C4).
+    The pressure is ill-conditioned: one refinement step moves the direct
+    solution's pressure by 3e-7 (C4) and 5e-7 (C1) relative. The FP64 CPU
+    oracle of this method reaches 7e-7 on C4's mean-free pressure at rtol
+    1e-14, held here to 2e-6. On C1 (contrast-1e3 inclusion, periodic in x)
+    the same oracle stays 2.5e-5 from the direct solution at rtol 1e-12 and at
+    1e-14 alike, with both solutions' residuals at round-off, and whether or
+    not the pinned pressure dof is counted: FP64 fixes the pressure there
+    only to that level (this GPU: 1.9e-5 FP64, 3.6e-5 mixed), held to 1e-4."""
     S = system(cfg, n)
     lin = fa.StokesLinearSystem(S.blocks, A_lof=kA, S_lof=0, precision=precision, rtol=1e-14, restart=60,
                                 inner_a_iters=60, max_outer=3000, block_hint=S.block_hint(hint),
@@ -212,7 +216,7 @@ def test_full_size_parity_against_direct_solve(cfg, n, kA, hint, precision):
     tp = t[S.n_u:]
     ep = np.linalg.norm(dp - dp.mean()) / np.linalg.norm(tp - tp.mean())
     assert eu < 1e-7, eu
-    assert ep < 2e-6, ep
+    assert ep < p_bar, ep
     lin.close()
 
 
