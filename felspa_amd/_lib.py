@@ -72,7 +72,7 @@ EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
     "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
-    "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_Tvmult", "fs_ilu_get_factors",
+    "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_vmult_noalloc", "fs_ilu_Tvmult", "fs_ilu_get_factors",
     "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
     "fs_system_create", "fs_system_destroy", "fs_system_get_info", "fs_system_csr",
@@ -119,6 +119,7 @@ def lib():
         "fs_ilu_create": (i32, [P(FsCsr), i32, i32, i32, P(i32), P(V)]),
         "fs_ilu_factorize": (i32, [V, P(f64)]),
         "fs_ilu_vmult": (i32, [V, P(f64), P(f64), i32]),
+        "fs_ilu_vmult_noalloc": (i32, [V, V, V, i32]),
         "fs_ilu_Tvmult": (i32, [V, P(f64), P(f64), i32]),
         "fs_ilu_get_factors": (i32, [V, P(i64), P(i64), P(i64), P(i32), P(f64), P(i32)]),
         "fs_ilu_destroy": (None, [V]),

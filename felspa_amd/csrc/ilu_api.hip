@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 
 #include "fs_stokes.h"
 #include "solver_internal.h"
@@ -18,6 +19,7 @@ using namespace fs;
 
 struct fs_ilu {
   int k = 0;
+  int device = 0;  // the GPU the factor lives on
   int64_t n = 0;
   hipStream_t st = nullptr;
   IluDev F;
@@ -113,6 +115,7 @@ int fs_ilu_create(const fs_csr* A, int32_t k, int32_t ordering, int32_t block_ro
   if (k < 0 || k > 20) throw FsError(FS_ERR_BAD_INPUT, "level of fill out of range");
   h->k = k;
   h->n = hA.n_rows;
+  hip_check(hipGetDevice(&h->device), "hipGetDevice");
   HostPattern pA = pattern_of(hA);
   Schedule s = ordering == FS_ORDER_REFERENCE
                  ? schedule_reference(h->n, 128)
@@ -224,6 +227,68 @@ static int ilu_apply(fs_ilu* h, const double* src, double* dst, int32_t precisio
 
 int fs_ilu_vmult(fs_ilu* h, const double* src, double* dst, int32_t precision) {
   return ilu_apply(h, src, dst, precision, false);
+}
+
+}  // extern "C"
+
+// ---- ILU application without allocation ------------------------------------
+// The two work vectors of fs_ilu_vmult_noalloc (the permuted right-hand side
+// and result, and the forward sweep's intermediate) in a file-scope device
+// array: every GPU the library runs on holds its own copy until the process
+// ends. Host code uses the address hipGetSymbolAddress returns for the
+// current device (taken once per device), never the array's name.
+constexpr int64_t FS_NOALLOC_MAX_ROWS = int64_t(1) << 21;  // 2,097,152 rows: 32 MiB
+static __device__ __align__(256) double scratch[2 * FS_NOALLOC_MAX_ROWS];
+static double* g_scratch_dev[64];
+static std::mutex g_scratch_mu;
+
+static double* scratch_for(int dev) {
+  if (dev < 0 || dev >= 64) throw FsError(FS_ERR_HIP, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw FsError(FS_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+extern "C" {
+
+int fs_ilu_vmult_noalloc(fs_ilu* h, const double* d_src, double* d_dst, int32_t precision) {
+  if (!h || !d_src || !d_dst) { fs::set_error("fs_ilu_vmult_noalloc: null argument"); return FS_ERR_BAD_INPUT; }
+  FS_GUARD_BEGIN
+  if (!h->factored) throw FsError(FS_ERR_BAD_INPUT, "fs_ilu: factorize before vmult");
+  if (precision != FS_PREC_FP64 && precision != FS_PREC_MIXED)
+    throw FsError(FS_ERR_BAD_INPUT, "fs_ilu_vmult_noalloc: precision must be FS_PREC_FP64 or FS_PREC_MIXED");
+  const int64_t n = h->n;
+  const int64_t ld = (n + 63) & ~int64_t(63);
+  if (ld > FS_NOALLOC_MAX_ROWS)
+    throw FsError(FS_ERR_UNSUPPORTED, "fs_ilu_vmult_noalloc: more than 2097152 rows (the static work area's size); "
+                                      "use fs_ilu_vmult");
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != h->device) throw FsError(FS_ERR_BAD_INPUT, "fs_ilu_vmult_noalloc: the factor lives on another device");
+  double* va = scratch_for(dev);  // permuted src, then the result
+  double* vb = va + ld;           // the forward sweep's intermediate
+  const int g = grid1(n);
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, h->perm.as<int32_t>(), d_src, va, 0);
+  if (precision == FS_PREC_MIXED) {
+    float* a = (float*)vb;
+    float* t = a + ld;
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(BLOCK), 0, h->st, n, (const double*)va, a);
+    h->F.apply<float, float, float>(a, t, a, h->st, h->cnt);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(BLOCK), 0, h->st, n, (const float*)a, va);
+  } else {
+    h->F.apply<double, double, double>(va, vb, va, h->st, h->cnt);
+  }
+  hipLaunchKernelGGL((k_permute<double>), dim3(g), dim3(BLOCK), 0, h->st, n, h->perm.as<int32_t>(),
+                     (const double*)va, d_dst, 1);
+  hip_check(hipGetLastError(), "ilu apply");
+  h->sync();
+  return FS_OK;
+  FS_GUARD_END
 }
 int fs_ilu_Tvmult(fs_ilu* h, const double* src, double* dst, int32_t precision) {
   return ilu_apply(h, src, dst, precision, true);

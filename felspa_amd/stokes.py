@@ -348,6 +348,12 @@ class SparseILU:
     def Tvmult(self, src, precision=L.FS_PREC_FP64):
         return self._apply(self._lib.fs_ilu_Tvmult, src, precision)
 
+    def vmult_device_noalloc(self, d_src_ptr, d_dst_ptr, precision=L.FS_PREC_FP64):
+        """vmult on FP64 device vectors (raw pointers) with the library's static
+        work area: no device allocation."""
+        L.check(self._lib.fs_ilu_vmult_noalloc(self.h, C.c_void_p(d_src_ptr), C.c_void_p(d_dst_ptr), precision),
+                "fs_ilu_vmult_noalloc")
+
     def factors(self, values=True):
         """(scipy CSR of the factor in its own row order, perm[new] = old)."""
         import scipy.sparse as sp

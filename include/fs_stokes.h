@@ -224,6 +224,15 @@ int fs_ilu_factorize(fs_ilu* ilu, const double* A_val);
 /* vmult (ilu.implement.h:544-595) / Tvmult (:626-680), host vectors in the
  * caller's order; precision FS_PREC_FP64 or FS_PREC_MIXED (FP32 factors). */
 int fs_ilu_vmult(fs_ilu* ilu, const double* src, double* dst, int32_t precision);
+/* fs_ilu_vmult on device vectors (FP64, caller's order, on the GPU that holds
+ * the factor) that allocates nothing: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops). Its two work vectors live in a static device area of
+ * 2 x 2,097,152 FP64 values (32 MiB per GPU, held until the process ends):
+ * factors of more rows return FS_ERR_UNSUPPORTED. The work area is shared by
+ * every call on a GPU, so calls that overlap on one GPU race; the call runs
+ * on the factor's stream and returns when it is done. */
+int fs_ilu_vmult_noalloc(fs_ilu* ilu, const double* d_src, double* d_dst, int32_t precision);
 int fs_ilu_Tvmult(fs_ilu* ilu, const double* src, double* dst, int32_t precision);
 /* Factor pattern and values in the factor's (permuted) order, rows sorted,
  * diagonal in place. Pass NULL arrays to query n / nnz. */
