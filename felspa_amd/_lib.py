@@ -71,13 +71,14 @@ class FsSystemInfo(C.Structure):
 EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
-    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
+    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_dof_lattice", "fs_stokes_velocity_lattice", "fs_stokes_velocity_at_points", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_vmult_noalloc", "fs_ilu_Tvmult", "fs_ilu_get_factors",
     "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
     "fs_system_create", "fs_system_destroy", "fs_system_get_info", "fs_system_csr",
     "fs_system_rhs", "fs_system_constraints", "fs_system_distribute", "fs_system_dof_maps",
     "fs_system_log_eta_grid", "fs_system_block_hint",
+    "fs_blocks_save", "fs_blocks_load", "fs_blocks_get", "fs_blocks_rhs", "fs_blocks_free",
     "fs_guess_model_create", "fs_guess_model_destroy", "fs_guess_model_get_config",
     "fs_guess_model_save", "fs_guess_model_load", "fs_guess_model_num_params",
     "fs_guess_model_get_params", "fs_guess_model_set_params", "fs_guess_model_get_grads",
@@ -109,6 +110,9 @@ def lib():
         "fs_stokes_set_guess_field": (i32, [V, V, P(f32), i32, P(i64), P(C.c_int8), P(i64)]),
         "fs_stokes_set_guess_mode": (i32, [V, i32]),
         "fs_stokes_set_precision": (i32, [V, i32]),
+        "fs_stokes_set_dof_lattice": (i32, [V, i32, i32, P(i64), P(C.c_int8)]),
+        "fs_stokes_velocity_lattice": (i32, [V, V, V]),
+        "fs_stokes_velocity_at_points": (i32, [V, V, i64, V, V]),
         "fs_stokes_set_method": (i32, [V, i32]),
         "fs_stokes_get_ordering": (i32, [V, i32, P(i32), P(i32)]),
         "fs_stokes_get_factor_info": (i32, [V, i32, P(i64), P(i64), P(i64), P(i32)]),
@@ -141,6 +145,11 @@ def lib():
         "fs_system_dof_maps": (i32, [V, P(P(i64)), P(P(C.c_int8)), P(P(i64))]),
         "fs_system_log_eta_grid": (P(f32), [V]),
         "fs_system_block_hint": (i32, [V, i32, P(i32)]),
+        "fs_blocks_save": (i32, [C.c_char_p] + [P(FsCsr)] * 5 + [P(f64)]),
+        "fs_blocks_load": (i32, [C.c_char_p, P(V)]),
+        "fs_blocks_get": (i32, [V, i32, P(FsCsr)]),
+        "fs_blocks_rhs": (P(f64), [V, P(i64), P(i64)]),
+        "fs_blocks_free": (None, [V]),
         "fs_guess_model_create": (i32, [P(FsGuessConfig), P(V)]),
         "fs_guess_model_destroy": (None, [V]),
         "fs_guess_model_get_config": (i32, [V, P(FsGuessConfig)]),

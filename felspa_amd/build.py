@@ -12,7 +12,7 @@ INC = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "libfelspa_amd.so")
 OBJ = os.path.join(HERE, "_build")
 
-HOST_SRC = ["assemble.cpp", "csr_util.cpp", "ordering.cpp"]
+HOST_SRC = ["assemble.cpp", "csr_util.cpp", "ordering.cpp", "blocks_io.cpp"]
 HIP_SRC = ["solver.hip", "stokes.hip", "ilu_api.hip", "guess.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -164,6 +164,58 @@ __global__ __launch_bounds__(BLOCK) void k_diag_rows(int64_t nr, const int32_t* 
   }
 }
 
+// Velocity hand-off (VelocityExtractor, stokes_common.implement.h:582-591):
+// u[vlat[i]] = x[i] for the velocity dofs i (caller's order); vlat = comp * P
+// + node on the Q2 node lattice of P = (2N+1)^d nodes, x fastest
+__global__ __launch_bounds__(BLOCK) void k_velocity_lattice(int64_t nu, const int32_t* __restrict__ vlat,
+                                                            const double* __restrict__ x, double* __restrict__ u) {
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < nu; i += (int64_t)gridDim.x * BLOCK)
+    u[vlat[i]] = x[i];
+}
+
+// The Q2 velocity field at points of [0,1]^D (FEValues get_function_values
+// on the uniform mesh of N^D cells): the point's cell, its local coordinates
+// t in [0,1]^D, and the tensor product of the 1D quadratic Lagrange basis on
+// the cell's nodes 0, 1/2, 1 (lattice offsets 0, 1, 2). Points outside the
+// unit cube are clamped to it. pts and vel: D values per point.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_velocity_at_points(int64_t np, const double* __restrict__ pts, int N,
+                                                              const double* __restrict__ u, int64_t P,
+                                                              double* __restrict__ vel) {
+  const int64_t n1 = 2 * (int64_t)N + 1;
+  for (int64_t q = (int64_t)blockIdx.x * BLOCK + threadIdx.x; q < np; q += (int64_t)gridDim.x * BLOCK) {
+    double l[D][3];
+    int64_t base = 0, stride = 1;
+    for (int a = 0; a < D; ++a) {
+      const double xa = fmin(fmax(pts[q * D + a], 0.0), 1.0) * N;
+      const int c = min((int)floor(xa), N - 1);
+      const double t = xa - c;
+      l[a][0] = 2.0 * (t - 0.5) * (t - 1.0);
+      l[a][1] = -4.0 * t * (t - 1.0);
+      l[a][2] = 2.0 * t * (t - 0.5);
+      base += 2 * (int64_t)c * stride;
+      stride *= n1;
+    }
+    double acc[D];
+    for (int c = 0; c < D; ++c) acc[c] = 0.0;
+    int o[3] = {0, 0, 0};
+    for (int k = 0; k < (D == 2 ? 9 : 27); ++k) {
+      int kk = k;
+      double w = 1.0;
+      int64_t idx = base, st2 = 1;
+      for (int a = 0; a < D; ++a) {
+        o[a] = kk % 3;
+        kk /= 3;
+        w *= l[a][o[a]];
+        idx += o[a] * st2;
+        st2 *= n1;
+      }
+      for (int c = 0; c < D; ++c) acc[c] += w * u[c * P + idx];
+    }
+    for (int c = 0; c < D; ++c) vel[q * D + c] = acc[c];
+  }
+}
+
 }  // namespace
 
 struct fs_handle {
@@ -191,6 +243,9 @@ struct fs_handle {
   fs_guess_model* model = nullptr;
   int guess_nc = 0;
   DevBuf g_logeta, g_idx, g_diag_rows;
+  // node lattice of the caller's velocity dofs (fs_stokes_set_dof_lattice)
+  int lat_dim = 0, lat_nc = 0;
+  DevBuf v_lat;
   std::vector<int32_t> diag_only;  // solver rows whose K row is the diagonal alone
 
   ~fs_handle() {
@@ -1374,6 +1429,66 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const 
   if (!h->diag_only.empty()) h->g_diag_rows.upload(h->diag_only.data(), 4 * h->diag_only.size());
   h->model = model;
   h->guess_nc = n_cells;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_set_dof_lattice(fs_handle* h, int32_t dim, int32_t n_cells, const int64_t* vnode,
+                              const int8_t* vcomp) {
+  if (!h || (dim != 2 && dim != 3) || n_cells < 1 || !vnode || !vcomp) {
+    fs::set_error("fs_stokes_set_dof_lattice: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  FS_GUARD_BEGIN
+  int64_t P = 1;
+  for (int a = 0; a < dim; ++a) P *= 2 * (int64_t)n_cells + 1;
+  if (dim * P >= INT32_MAX) throw FsError(FS_ERR_BAD_INPUT, "lattice too large");
+  std::vector<int32_t> idx(h->nu);
+  std::vector<uint8_t> seen((size_t)(dim * P), 0);
+  for (int64_t i = 0; i < h->nu; ++i) {
+    if (vnode[i] < 0 || vnode[i] >= P || vcomp[i] < 0 || vcomp[i] >= dim)
+      throw FsError(FS_ERR_BAD_INPUT, "velocity dof map out of range");
+    idx[i] = (int32_t)(vcomp[i] * P + vnode[i]);
+    if (seen[idx[i]]++) throw FsError(FS_ERR_BAD_INPUT, "two velocity dofs on one lattice node and component");
+  }
+  if (h->nu != dim * P) throw FsError(FS_ERR_BAD_INPUT, "the velocity dofs do not cover the node lattice");
+  h->v_lat.upload(idx.data(), 4 * idx.size());
+  h->lat_dim = dim;
+  h->lat_nc = n_cells;
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_velocity_lattice(fs_handle* h, const double* d_x, double* d_u) {
+  if (!h || !d_x || !d_u) { fs::set_error("fs_stokes_velocity_lattice: null argument"); return FS_ERR_BAD_INPUT; }
+  FS_GUARD_BEGIN
+  if (!h->lat_dim) throw FsError(FS_ERR_BAD_INPUT, "fs_stokes_set_dof_lattice first");
+  hipLaunchKernelGGL(k_velocity_lattice, dim3(h->grid(h->nu)), dim3(BLOCK), 0, h->st, h->nu, h->v_lat.as<int32_t>(),
+                     d_x, d_u);
+  hip_check(hipGetLastError(), "velocity lattice");
+  h->sync();
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_velocity_at_points(fs_handle* h, const double* d_u, int64_t n_points, const double* d_points,
+                                 double* d_vel) {
+  if (!h || !d_u || n_points < 0 || (n_points > 0 && (!d_points || !d_vel))) {
+    fs::set_error("fs_stokes_velocity_at_points: bad argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  FS_GUARD_BEGIN
+  if (!h->lat_dim) throw FsError(FS_ERR_BAD_INPUT, "fs_stokes_set_dof_lattice first");
+  if (n_points == 0) return FS_OK;
+  const int64_t P = h->nu / h->lat_dim;
+  if (h->lat_dim == 2)
+    hipLaunchKernelGGL(k_velocity_at_points<2>, dim3(h->grid(n_points)), dim3(BLOCK), 0, h->st, n_points, d_points,
+                       h->lat_nc, d_u, P, d_vel);
+  else
+    hipLaunchKernelGGL(k_velocity_at_points<3>, dim3(h->grid(n_points)), dim3(BLOCK), 0, h->st, n_points, d_points,
+                       h->lat_nc, d_u, P, d_vel);
+  hip_check(hipGetLastError(), "velocity at points");
+  h->sync();
   return FS_OK;
   FS_GUARD_END
 }

@@ -259,6 +259,28 @@ class StokesLinearSystem:
                                                  stg.ctypes.data_as(L.P(L.i32)), None), "fs_stokes_get_schedule")
         return blk, stg
 
+    def set_dof_lattice(self, system):
+        """The caller's velocity dofs on the Q2 node lattice of `system`
+        (felspa_amd.StokesSystem: its dimension, cells and dof maps)."""
+        self._vnode = np.ascontiguousarray(system.vnode, np.int64)
+        self._vcomp = np.ascontiguousarray(system.vcomp, np.int8)
+        self.lattice = (system.dim, system.n_cells)
+        L.check(self._lib.fs_stokes_set_dof_lattice(self.h, system.dim, system.n_cells,
+                                                    self._vnode.ctypes.data_as(L.P(L.i64)),
+                                                    self._vcomp.ctypes.data_as(L.P(C.c_int8))),
+                "fs_stokes_set_dof_lattice")
+
+    def velocity_lattice(self, d_x_ptr, d_u_ptr):
+        """VelocityExtractor: the velocity of device solution d_x on the node lattice (device d_u)."""
+        L.check(self._lib.fs_stokes_velocity_lattice(self.h, C.c_void_p(d_x_ptr), C.c_void_p(d_u_ptr)),
+                "fs_stokes_velocity_lattice")
+
+    def velocity_at_points(self, d_u_ptr, n_points, d_points_ptr, d_vel_ptr):
+        """The Q2 velocity field d_u at n_points device points (dim values each)."""
+        L.check(self._lib.fs_stokes_velocity_at_points(self.h, C.c_void_p(d_u_ptr), int(n_points),
+                                                       C.c_void_p(d_points_ptr), C.c_void_p(d_vel_ptr)),
+                "fs_stokes_velocity_at_points")
+
     def set_precision(self, precision):
         """FS_PREC_* for the next solves on the same factorisation."""
         L.check(self._lib.fs_stokes_set_precision(self.h, int(precision)), "fs_stokes_set_precision")

@@ -1,6 +1,7 @@
 """Q2-Q1 Stokes systems from the in-library generator (include/fs_assemble.h):
 blocks A, Bt, B, K11, Mp as scipy CSR, rhs, constraints and lattice maps."""
 import ctypes as C
+import os
 
 import numpy as np
 import scipy.sparse as sp
@@ -83,3 +84,48 @@ class StokesSystem:
                                                 out.ctypes.data_as(L.P(L.i32))),
                 "fs_system_block_hint")
         return out
+
+
+def save_blocks(path, blocks, rhs=None):
+    """fs_blocks_save: the five blocks (scipy CSR, as stored) and optionally
+    the rhs in the binary interchange format (include/fs_blocks_io.h)."""
+    lib = L.lib()
+    keep = []
+    cs = []
+    for k in BLOCK_NAMES:
+        m = blocks[k]
+        rp = np.ascontiguousarray(m.indptr, np.int64)
+        ci = np.ascontiguousarray(m.indices, np.int32)
+        v = np.ascontiguousarray(m.data, np.float64)
+        keep += [rp, ci, v]
+        cs.append(L.FsCsr(m.shape[0], m.shape[1], rp.ctypes.data_as(L.P(L.i64)), ci.ctypes.data_as(L.P(L.i32)),
+                          v.ctypes.data_as(L.P(L.f64)), 0))
+    r = None
+    if rhs is not None:
+        rr = np.ascontiguousarray(rhs, np.float64)
+        keep.append(rr)
+        r = rr.ctypes.data_as(L.P(L.f64))
+    L.check(lib.fs_blocks_save(os.fsencode(path), *[C.byref(c) for c in cs], r), "fs_blocks_save")
+
+
+def load_blocks(path):
+    """fs_blocks_load: (dict of scipy CSR blocks A, Bt, B, K11, Mp; rhs or None)."""
+    lib = L.lib()
+    h = C.c_void_p()
+    L.check(lib.fs_blocks_load(os.fsencode(path), C.byref(h)), "fs_blocks_load")
+    try:
+        out = {}
+        for w, k in enumerate(BLOCK_NAMES):
+            c = L.FsCsr()
+            L.check(lib.fs_blocks_get(h, w, C.byref(c)), "fs_blocks_get")
+            nnz = c.row_ptr[c.n_rows]
+            rp = np.ctypeslib.as_array(c.row_ptr, (c.n_rows + 1,)).copy()
+            ci = np.ctypeslib.as_array(c.col_idx, (max(nnz, 1),))[:nnz].copy()
+            v = np.ctypeslib.as_array(c.val, (max(nnz, 1),))[:nnz].copy()
+            out[k] = sp.csr_matrix((v, ci, rp), shape=(c.n_rows, c.n_cols))
+        nu, npp = L.i64(), L.i64()
+        r = lib.fs_blocks_rhs(h, C.byref(nu), C.byref(npp))
+        rhs = np.ctypeslib.as_array(r, (nu.value + npp.value,)).copy() if r else None
+        return out, rhs
+    finally:
+        lib.fs_blocks_free(h)

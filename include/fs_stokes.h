@@ -162,6 +162,25 @@ int fs_stokes_set_guess_field(fs_handle* h, struct fs_guess_model* model, const 
                               int32_t n_cells, const int64_t* vnode, const int8_t* vcomp,
                               const int64_t* pnode);
 
+/* Velocity hand-off to the level-set side (VelocityExtractor::extract,
+ * include/felspa/pde/src/stokes_common.implement.h:582-591, which evaluates the
+ * velocity of the solution at the points of an FEValues object):
+ * fs_stokes_set_dof_lattice places the caller's velocity dofs on the Q2 node
+ * lattice of the uniform mesh of n_cells^dim cells of [0,1]^dim (vnode [n_u]
+ * node index, x fastest, on the (2 n_cells + 1)^dim lattice; vcomp [n_u] its
+ * component; every node and component exactly once, e.g. fs_system_dof_maps);
+ * fs_stokes_velocity_lattice writes the velocity of a device solution vector
+ * d_x [n_u + n_p] (caller's order, constraints distributed) to d_u
+ * [dim][(2 n_cells + 1)^dim] in device memory; fs_stokes_velocity_at_points
+ * evaluates that Q2 field at n_points device points d_points [n_points][dim]
+ * (clamped to the unit cube) into d_vel [n_points][dim]. The velocity stays in
+ * HBM for a device-side advection step. */
+int fs_stokes_set_dof_lattice(fs_handle* h, int32_t dim, int32_t n_cells, const int64_t* vnode,
+                              const int8_t* vcomp);
+int fs_stokes_velocity_lattice(fs_handle* h, const double* d_x, double* d_u);
+int fs_stokes_velocity_at_points(fs_handle* h, const double* d_u, int64_t n_points, const double* d_points,
+                                 double* d_vel);
+
 /* Switch the solution method (fs_opts.method) of an existing handle between
  * solves: StokesSolutionMethod FC / SCR (stokes_common.h:323-328) on the same
  * factorisation; CompareTest (stokes_ilu.cc:471-505) is two solves and a

@@ -244,3 +244,22 @@ def test_user_guess_matches_oracle(oracle):
     xo1, sto1 = oracle.stokes_solve(S.blocks, S.rhs, kA=1, kS=0)
     assert abs(st1["outer_iters"] - sto1["outer_iters"]) <= 1
     lin.close()
+
+
+def test_system_from_block_file_solves_identically(tmp_path):
+    """A system written and read back through the block-CSR interchange
+    (include/fs_blocks_io.h) gives the same solve bit for bit."""
+    S = system("C2", 8)
+    path = str(tmp_path / "c2.fsbk")
+    fa.save_blocks(path, S.blocks, S.rhs)
+    blocks, rhs = fa.load_blocks(path)
+    xs = []
+    for bl, b in ((S.blocks, S.rhs), (blocks, rhs)):
+        lin = fa.StokesLinearSystem(bl, A_lof=1, precision=L.FS_PREC_MIXED, block_hint=S.block_hint(5),
+                                    constraints=S.constrained, restart=60, inner_a_iters=30)
+        x = np.zeros(S.n)
+        st = lin.solve(x, b)
+        assert st["true_rel_residual"] <= 1e-8
+        xs.append(x)
+        lin.close()
+    assert np.array_equal(xs[0], xs[1])
