@@ -6,9 +6,9 @@ from .configs import CONFIGS, make_problem, solver_settings
 from .stokes import (FC, SCR, CompareTest, NoConvergence, SparseILU, SparseMatrix,
                      StokesLinearSystem, StokesSolverControl, block_color_ordering,
                      ilu_symbolic)
-from .system import StokesSystem, load_blocks, problem, save_blocks
+from .system import DeviceAssembler, StokesSystem, load_blocks, problem, save_blocks
 
 __all__ = ["StokesLinearSystem", "StokesSolverControl", "SparseILU", "SparseMatrix",
-           "NoConvergence", "StokesSystem", "problem", "CONFIGS", "make_problem",
+           "NoConvergence", "StokesSystem", "DeviceAssembler", "problem", "CONFIGS", "make_problem",
            "solver_settings", "ilu_symbolic", "block_color_ordering", "save_blocks", "load_blocks", "FC", "SCR",
            "CompareTest", "_lib"]

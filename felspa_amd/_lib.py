@@ -69,7 +69,7 @@ class FsSystemInfo(C.Structure):
 
 # every symbol the public headers declare (checked by the CPU test suite)
 EXPORTS = [
-    "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_solve",
+    "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_refactor_device", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
     "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_dof_lattice", "fs_stokes_velocity_lattice", "fs_stokes_velocity_at_points", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_vmult_noalloc", "fs_ilu_Tvmult", "fs_ilu_get_factors",
@@ -77,7 +77,9 @@ EXPORTS = [
     "fs_matrix_destroy", "fs_set_device_memory_cap",
     "fs_system_create", "fs_system_destroy", "fs_system_get_info", "fs_system_csr",
     "fs_system_rhs", "fs_system_constraints", "fs_system_distribute", "fs_system_dof_maps",
-    "fs_system_log_eta_grid", "fs_system_block_hint",
+    "fs_system_log_eta_grid", "fs_system_block_hint", "fs_system_quadrature_fields",
+    "fs_assembler_create", "fs_assembler_sample_field", "fs_assembler_assemble", "fs_assembler_last_ms",
+    "fs_assembler_destroy",
     "fs_blocks_save", "fs_blocks_load", "fs_blocks_get", "fs_blocks_rhs", "fs_blocks_free",
     "fs_guess_model_create", "fs_guess_model_destroy", "fs_guess_model_get_config",
     "fs_guess_model_save", "fs_guess_model_load", "fs_guess_model_num_params",
@@ -103,6 +105,7 @@ def lib():
         "fs_opts_default": (None, [P(FsOpts)]),
         "fs_stokes_create": (i32, [P(FsCsr)] * 5 + [P(FsOpts), P(V)]),
         "fs_stokes_refactor": (i32, [V] + [P(f64)] * 5),
+        "fs_stokes_refactor_device": (i32, [V] * 6),
         "fs_stokes_solve": (i32, [V, P(f64), P(f64), P(FsStats)]),
         "fs_stokes_solve_device": (i32, [V, V, V, P(FsStats)]),
         "fs_stokes_vmult": (i32, [V, P(f64), P(f64), i32]),
@@ -145,6 +148,12 @@ def lib():
         "fs_system_dof_maps": (i32, [V, P(P(i64)), P(P(C.c_int8)), P(P(i64))]),
         "fs_system_log_eta_grid": (P(f32), [V]),
         "fs_system_block_hint": (i32, [V, i32, P(i32)]),
+        "fs_system_quadrature_fields": (i32, [V, P(f64), P(f64)]),
+        "fs_assembler_create": (i32, [V, P(V)]),
+        "fs_assembler_sample_field": (i32, [V, V, V]),
+        "fs_assembler_assemble": (i32, [V] * 9),
+        "fs_assembler_last_ms": (C.c_double, [V]),
+        "fs_assembler_destroy": (None, [V]),
         "fs_blocks_save": (i32, [C.c_char_p] + [P(FsCsr)] * 5 + [P(f64)]),
         "fs_blocks_load": (i32, [C.c_char_p, P(V)]),
         "fs_blocks_get": (i32, [V, i32, P(FsCsr)]),

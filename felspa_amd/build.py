@@ -13,14 +13,14 @@ OUT = os.path.join(HERE, "libfelspa_amd.so")
 OBJ = os.path.join(HERE, "_build")
 
 HOST_SRC = ["assemble.cpp", "csr_util.cpp", "ordering.cpp", "blocks_io.cpp"]
-HIP_SRC = ["solver.hip", "stokes.hip", "ilu_api.hip", "guess.hip"]
+HIP_SRC = ["solver.hip", "stokes.hip", "ilu_api.hip", "guess.hip", "assemble_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              # packed FP32 instructions stay off (v_pk_*_f32)
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", INC, "-I", CSRC]
-HOST_FLAGS = ["-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-std=c++17", "-Wall",
+HOST_FLAGS = ["-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-std=c++17", "-Wall",
               "-I", INC, "-I", CSRC]
 
 

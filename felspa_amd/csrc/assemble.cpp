@@ -23,6 +23,7 @@
 #include <new>
 #include <stdexcept>
 
+#include "assemble_internal.h"
 #include "fs_assemble.h"
 #include "fs_common.h"
 #include "fs_stokes.h"
@@ -65,46 +66,54 @@ struct Xoshiro256ss {
   double uniform(double a, double b) { return a + (b - a) * uniform(); }
 };
 
-struct Ellipsoid {
-  double c[3];
-  double ax[3];
-  double R[3][3];  // rows: body axes in world coordinates
-  double logc;
-};
+}  // namespace
+
+namespace fs {
+
+std::vector<FieldEllipsoid> field_ellipsoids(const fs_problem& p) {
+  std::vector<FieldEllipsoid> ell;
+  if (p.field != FS_FIELD_ELLIPSES) return ell;
+  Xoshiro256ss rng(p.seed);
+  int nmin = std::max(1, p.n_ellipses_min), nmax = std::max(nmin, p.n_ellipses_max);
+  int n = nmin + (int)(rng.next() % (uint64_t)(nmax - nmin + 1));
+  for (int e = 0; e < n; ++e) {
+    FieldEllipsoid E{};
+    for (int k = 0; k < p.dim; ++k) E.c[k] = rng.uniform();
+    for (int k = 0; k < p.dim; ++k) E.ax[k] = rng.uniform(p.axis_min, p.axis_max);
+    if (p.dim == 2) {
+      double th = rng.uniform(0.0, M_PI);
+      double cs = std::cos(th), sn = std::sin(th);
+      E.R[0][0] = cs; E.R[0][1] = sn; E.R[0][2] = 0;
+      E.R[1][0] = -sn; E.R[1][1] = cs; E.R[1][2] = 0;
+      E.R[2][0] = 0; E.R[2][1] = 0; E.R[2][2] = 1;
+    } else {
+      // uniform random rotation from a uniform unit quaternion (Shoemake 1992)
+      double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+      double a = std::sqrt(1 - u1), b = std::sqrt(u1);
+      double w = a * std::sin(2 * M_PI * u2), x = a * std::cos(2 * M_PI * u2);
+      double y = b * std::sin(2 * M_PI * u3), z = b * std::cos(2 * M_PI * u3);
+      E.R[0][0] = 1 - 2 * (y * y + z * z); E.R[0][1] = 2 * (x * y - z * w); E.R[0][2] = 2 * (x * z + y * w);
+      E.R[1][0] = 2 * (x * y + z * w); E.R[1][1] = 1 - 2 * (x * x + z * z); E.R[1][2] = 2 * (y * z - x * w);
+      E.R[2][0] = 2 * (x * z - y * w); E.R[2][1] = 2 * (y * z + x * w); E.R[2][2] = 1 - 2 * (x * x + y * y);
+    }
+    E.logc = rng.uniform(p.logc_min, p.logc_max);
+    E.scale = std::pow(10.0, E.logc);
+    ell.push_back(E);
+  }
+  return ell;
+}
+
+double layer_interface(const fs_problem& p, double x0) { return 0.5 + p.amp * std::cos(2 * M_PI * x0); }
+
+}  // namespace fs
+
+namespace {
 
 struct Field {
   fs_problem p;
-  std::vector<Ellipsoid> ell;
+  std::vector<fs::FieldEllipsoid> ell;
 
-  void init() {
-    if (p.field != FS_FIELD_ELLIPSES) return;
-    Xoshiro256ss rng(p.seed);
-    int nmin = std::max(1, p.n_ellipses_min), nmax = std::max(nmin, p.n_ellipses_max);
-    int n = nmin + (int)(rng.next() % (uint64_t)(nmax - nmin + 1));
-    for (int e = 0; e < n; ++e) {
-      Ellipsoid E{};
-      for (int k = 0; k < p.dim; ++k) E.c[k] = rng.uniform();
-      for (int k = 0; k < p.dim; ++k) E.ax[k] = rng.uniform(p.axis_min, p.axis_max);
-      if (p.dim == 2) {
-        double th = rng.uniform(0.0, M_PI);
-        double cs = std::cos(th), sn = std::sin(th);
-        E.R[0][0] = cs; E.R[0][1] = sn; E.R[0][2] = 0;
-        E.R[1][0] = -sn; E.R[1][1] = cs; E.R[1][2] = 0;
-        E.R[2][0] = 0; E.R[2][1] = 0; E.R[2][2] = 1;
-      } else {
-        // uniform random rotation from a uniform unit quaternion (Shoemake 1992)
-        double u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
-        double a = std::sqrt(1 - u1), b = std::sqrt(u1);
-        double w = a * std::sin(2 * M_PI * u2), x = a * std::cos(2 * M_PI * u2);
-        double y = b * std::sin(2 * M_PI * u3), z = b * std::cos(2 * M_PI * u3);
-        E.R[0][0] = 1 - 2 * (y * y + z * z); E.R[0][1] = 2 * (x * y - z * w); E.R[0][2] = 2 * (x * z + y * w);
-        E.R[1][0] = 2 * (x * y + z * w); E.R[1][1] = 1 - 2 * (x * x + z * z); E.R[1][2] = 2 * (y * z - x * w);
-        E.R[2][0] = 2 * (x * z - y * w); E.R[2][1] = 2 * (y * z + x * w); E.R[2][2] = 1 - 2 * (x * x + y * y);
-      }
-      E.logc = rng.uniform(p.logc_min, p.logc_max);
-      ell.push_back(E);
-    }
-  }
+  void init() { ell = fs::field_ellipsoids(p); }
 
   // viscosity and density at a point
   void eval(const double* x, double& eta, double& rho) const {
@@ -131,13 +140,13 @@ struct Field {
         return;
       }
       case FS_FIELD_LAYERS: {
-        double yif = 0.5 + p.amp * std::cos(2 * M_PI * x[0]);
+        double yif = fs::layer_interface(p, x[0]);
         if (x[d - 1] > yif) { eta = p.eta_incl; rho = p.rho_incl; }
         else { eta = p.eta_matrix; rho = p.rho_matrix; }
         return;
       }
       case FS_FIELD_ELLIPSES: {
-        double best = -1.0;
+        double best = -1.0, scale = 1.0;
         for (const auto& E : ell) {
           double r2 = 0;
           for (int a = 0; a < d; ++a) {
@@ -145,9 +154,9 @@ struct Field {
             for (int k = 0; k < d; ++k) s += E.R[a][k] * (x[k] - E.c[k]);
             r2 += (s / E.ax[a]) * (s / E.ax[a]);
           }
-          if (r2 <= 1.0 && E.logc > best) best = E.logc;
+          if (r2 <= 1.0 && E.logc > best) { best = E.logc; scale = E.scale; }
         }
-        eta = best >= 0 ? p.eta_matrix * std::pow(10.0, best) : p.eta_matrix;
+        eta = best >= 0 ? p.eta_matrix * scale : p.eta_matrix;
         rho = p.rho_matrix;
         return;
       }
@@ -230,75 +239,69 @@ std::vector<int64_t> cuthill_mckee(const Lattice& L, bool q2) {
   return order;
 }
 
-// ---------------------------------------------------------------------------
-// Reference-element data on the unit cell, scaled to cell size h.
-struct RefElement {
-  int d, nv, np, nq;  // velocity nodes per cell (3^d), pressure nodes (2^d), quad points (3^d)
-  std::vector<double> N;   // [nq][nv]
-  std::vector<double> dN;  // [nq][nv][d]   physical gradient
-  std::vector<double> P;   // [nq][np]
-  std::vector<double> JxW; // [nq]
-  std::vector<double> xq;  // [nq][d] local coordinates in [0,1]
-
-  RefElement(int d_, double h) : d(d_) {
-    const double g = std::sqrt(3.0 / 5.0) / 2.0;
-    const double qp[3] = {0.5 - g, 0.5, 0.5 + g};
-    const double qw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
-    nv = 1; np = 1; nq = 1;
-    for (int k = 0; k < d; ++k) { nv *= 3; np *= 2; nq *= 3; }
-    N.resize((size_t)nq * nv);
-    dN.resize((size_t)nq * nv * d);
-    P.resize((size_t)nq * np);
-    JxW.resize(nq);
-    xq.resize((size_t)nq * d);
-    auto L2 = [](int a, double x) {
-      return a == 0 ? 2 * (x - 0.5) * (x - 1) : a == 1 ? -4 * x * (x - 1) : 2 * x * (x - 0.5);
-    };
-    auto dL2 = [](int a, double x) { return a == 0 ? 4 * x - 3 : a == 1 ? -8 * x + 4 : 4 * x - 1; };
-    auto L1 = [](int b, double x) { return b == 0 ? 1 - x : x; };
-    double hd = 1;
-    for (int k = 0; k < d; ++k) hd *= h;
-    for (int q = 0; q < nq; ++q) {
-      int qi[3] = {q % 3, (q / 3) % 3, (q / 9) % 3};
-      double x[3] = {0, 0, 0}, w = 1;
-      for (int k = 0; k < d; ++k) { x[k] = qp[qi[k]]; w *= qw[qi[k]]; xq[(size_t)q * d + k] = x[k]; }
-      JxW[q] = w * hd;
-      for (int n = 0; n < nv; ++n) {
-        int a[3] = {n % 3, (n / 3) % 3, (n / 9) % 3};
-        double val = 1;
-        for (int k = 0; k < d; ++k) val *= L2(a[k], x[k]);
-        N[(size_t)q * nv + n] = val;
-        for (int c = 0; c < d; ++c) {
-          double gv = 1;
-          for (int k = 0; k < d; ++k) gv *= (k == c) ? dL2(a[k], x[k]) / h : L2(a[k], x[k]);
-          dN[((size_t)q * nv + n) * d + c] = gv;
-        }
-      }
-      for (int m = 0; m < np; ++m) {
-        int b[3] = {m % 2, (m / 2) % 2, (m / 4) % 2};
-        double val = 1;
-        for (int k = 0; k < d; ++k) val *= L1(b[k], x[k]);
-        P[(size_t)q * np + m] = val;
-      }
-    }
-  }
-};
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
-struct fs_system {
-  int dim = 0, N = 0;
-  int64_t n_u = 0, n_p = 0;
-  HostCsr A, Bt, B, K11, Mp;
-  std::vector<double> rhs;
-  std::vector<int8_t> kind;
-  std::vector<int64_t> master;
-  std::vector<double> cval;
-  std::vector<int64_t> vnode, pnode;
-  std::vector<int8_t> vcomp;
-  std::vector<float> log_eta;
-};
+// Reference-element data on the unit cell, scaled to cell size h.
+namespace fs {
+
+RefTables ref_tables(int d, double h) {
+  RefTables R;
+  R.d = d;
+  const double g = std::sqrt(3.0 / 5.0) / 2.0;
+  const double qp[3] = {0.5 - g, 0.5, 0.5 + g};
+  const double qw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+  int nv = 1, np = 1, nq = 1;
+  for (int k = 0; k < d; ++k) { nv *= 3; np *= 2; nq *= 3; }
+  R.nv = nv; R.np = np; R.nq = nq;
+  R.N.resize((size_t)nq * nv);
+  R.dN.resize((size_t)nq * nv * d);
+  R.P.resize((size_t)nq * np);
+  R.JxW.resize(nq);
+  R.xq.resize((size_t)nq * d);
+  auto L2 = [](int a, double x) {
+    return a == 0 ? 2 * (x - 0.5) * (x - 1) : a == 1 ? -4 * x * (x - 1) : 2 * x * (x - 0.5);
+  };
+  auto dL2 = [](int a, double x) { return a == 0 ? 4 * x - 3 : a == 1 ? -8 * x + 4 : 4 * x - 1; };
+  auto L1 = [](int b, double x) { return b == 0 ? 1 - x : x; };
+  double hd = 1;
+  for (int k = 0; k < d; ++k) hd *= h;
+  for (int q = 0; q < nq; ++q) {
+    int qi[3] = {q % 3, (q / 3) % 3, (q / 9) % 3};
+    double x[3] = {0, 0, 0}, w = 1;
+    for (int k = 0; k < d; ++k) { x[k] = qp[qi[k]]; w *= qw[qi[k]]; R.xq[(size_t)q * d + k] = x[k]; }
+    R.JxW[q] = w * hd;
+    for (int n = 0; n < nv; ++n) {
+      int a[3] = {n % 3, (n / 3) % 3, (n / 9) % 3};
+      double val = 1;
+      for (int k = 0; k < d; ++k) val *= L2(a[k], x[k]);
+      R.N[(size_t)q * nv + n] = val;
+      for (int c = 0; c < d; ++c) {
+        double gv = 1;
+        for (int k = 0; k < d; ++k) gv *= (k == c) ? dL2(a[k], x[k]) / h : L2(a[k], x[k]);
+        R.dN[((size_t)q * nv + n) * d + c] = gv;
+      }
+    }
+    for (int m = 0; m < np; ++m) {
+      int b[3] = {m % 2, (m / 2) % 2, (m / 4) % 2};
+      double val = 1;
+      for (int k = 0; k < d; ++k) val *= L1(b[k], x[k]);
+      R.P[(size_t)q * np + m] = val;
+    }
+  }
+  // Bl[j][(n,c)] = -sum_q psi_j dN_n/dx_c JxW  (viscosity independent)
+  const int nvl = nv * d;
+  R.Bl.assign((size_t)np * nvl, 0.0);
+  for (int q = 0; q < nq; ++q)
+    for (int j = 0; j < np; ++j)
+      for (int n = 0; n < nv; ++n)
+        for (int c = 0; c < d; ++c)
+          R.Bl[(size_t)j * nvl + n * d + c] -=
+            R.P[(size_t)q * np + j] * R.dN[((size_t)q * nv + n) * d + c] * R.JxW[q];
+  return R;
+}
+
+}  // namespace fs
 
 namespace {
 
@@ -314,6 +317,7 @@ void build(const fs_problem& prob, fs_system& S) {
   const int64_t nvn = LV.size(), npn = LP.size();
   S.dim = d;
   S.N = (int)Nc;
+  S.prob = prob;
   S.n_u = nvn * d;
   S.n_p = npn;
   const int64_t n_u = S.n_u, n_p = S.n_p, ntot = n_u + n_p;
@@ -541,7 +545,7 @@ void build(const fs_problem& prob, fs_system& S) {
   for (int64_t q = 0; q < n_p; ++q) S.K11.ci[q] = (int32_t)q;
 
   // ---------------- values ----------------
-  RefElement R(d, h);
+  const fs::RefTables R = fs::ref_tables(d, h);
   const int nvl = R.nv * d;  // local velocity dofs
   // G_q[(n,c),(m,e)] = (delta_ce gradN_n.gradN_m + dN_n/dx_e dN_m/dx_c) JxW
   std::vector<double> G((size_t)R.nq * nvl * nvl);
@@ -557,14 +561,7 @@ void build(const fs_problem& prob, fs_system& S) {
             double v = (c == e ? dot : 0.0) + gn[e] * gm[c];
             G[((size_t)q * nvl + n * d + c) * nvl + m * d + e] = v * R.JxW[q];
           }
-  // Bl[j][(n,c)] = -sum_q psi_j dN_n/dx_c JxW  (viscosity independent)
-  std::vector<double> Bl((size_t)R.np * nvl, 0.0);
-  for (int q = 0; q < R.nq; ++q)
-    for (int j = 0; j < R.np; ++j)
-      for (int n = 0; n < R.nv; ++n)
-        for (int c = 0; c < d; ++c)
-          Bl[(size_t)j * nvl + n * d + c] -=
-            R.P[(size_t)q * R.np + j] * R.dN[((size_t)q * R.nv + n) * d + c] * R.JxW[q];
+  const std::vector<double>& Bl = R.Bl;
 
   std::vector<double> rhs(ntot, 0.0), own_diag(ntot, 0.0);
   auto find = [](const HostCsr& M, int64_t r, int64_t c) -> int64_t {
@@ -676,10 +673,16 @@ void build(const fs_problem& prob, fs_system& S) {
     }
   }
   // constrained rows
+  // mean |A_ii| over free velocity rows, summed by row chunks (the device
+  // assembler sums in the same order)
   double diag_sum = 0;
   int64_t diag_cnt = 0;
-  for (int64_t r = 0; r < n_u; ++r)
-    if (eq[r] == r) { diag_sum += std::fabs(S.A.v[find(S.A, r, r)]); ++diag_cnt; }
+  for (int64_t r0 = 0; r0 < n_u; r0 += fs::DIAG_CHUNK) {
+    double part = 0;
+    for (int64_t r = r0; r < std::min(n_u, r0 + fs::DIAG_CHUNK); ++r)
+      if (eq[r] == r) { part += std::fabs(S.A.v[find(S.A, r, r)]); ++diag_cnt; }
+    diag_sum += part;
+  }
   const double Dp = diag_cnt ? diag_sum / (double)diag_cnt : 1.0;
   for (int64_t r = 0; r < n_u; ++r) {
     if (eq[r] == r) continue;
@@ -806,6 +809,30 @@ int fs_system_dof_maps(const fs_system* S, const int64_t** vnode, const int8_t**
 }
 
 const float* fs_system_log_eta_grid(const fs_system* S) { return S ? S->log_eta.data() : nullptr; }
+
+int fs_system_quadrature_fields(const fs_system* S, double* eta_q, double* rho_q) {
+  if (!S || !eta_q || !rho_q) { fs::set_error("fs_system_quadrature_fields: null argument"); return FS_ERR_BAD_INPUT; }
+  const int d = S->dim;
+  const int64_t Nc = S->N;
+  const double h = 1.0 / (double)Nc;
+  const fs::RefTables R = fs::ref_tables(d, h);
+  Field F;
+  F.p = S->prob;
+  F.init();
+  int64_t ncell = 1;
+  for (int k = 0; k < d; ++k) ncell *= Nc;
+#pragma omp parallel for schedule(static)
+  for (int64_t cell = 0; cell < ncell; ++cell) {
+    int64_t cc[3] = {0, 0, 0}, t = cell;
+    for (int k = 0; k < d; ++k) { cc[k] = t % Nc; t /= Nc; }
+    for (int q = 0; q < R.nq; ++q) {
+      double x[3] = {0, 0, 0};
+      for (int k = 0; k < d; ++k) x[k] = (cc[k] + R.xq[(size_t)q * d + k]) * h;
+      F.eval(x, eta_q[cell * R.nq + q], rho_q[cell * R.nq + q]);
+    }
+  }
+  return FS_OK;
+}
 
 int fs_system_block_hint(const fs_system* S, int32_t nodes_per_block, int32_t* out) {
   if (!S || !out || nodes_per_block < 1) { fs::set_error("fs_system_block_hint: bad argument"); return FS_ERR_BAD_INPUT; }

@@ -1096,6 +1096,41 @@ void BlkCsr::upload_values(const double* vals, hipStream_t st) {
   hip_check(hipStreamSynchronize(st), "sync");
 }
 
+// dst[map[e]] = src[e]
+static __global__ __launch_bounds__(BLOCK) void k_scatter_map(int64_t n, const int64_t* __restrict__ map,
+                                                              const double* __restrict__ src,
+                                                              double* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * BLOCK)
+    dst[map[e]] = src[e];
+}
+
+void BlkCsr::upload_values_device(const double* d_vals, hipStream_t st) {
+  if (nb == 0) return;
+  if (!d_pos.p) {
+    d_pos.upload(pos.data(), 8 * std::max<int64_t>(1, nnz));
+    hip_check(hipMemsetAsync(v64.p, 0, 8 * pad.size(), st), "memset");  // padding entries stay 0
+  }
+  hipLaunchKernelGGL(k_scatter_map, dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, d_pos.as<int64_t>(), d_vals,
+                     v64.as<double>());
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad.size())), dim3(BLOCK), 0, st,
+                     (int64_t)pad.size(), v64.as<double>(), v32.as<float>());
+}
+
+void DevCsr::derive_from_v64(hipStream_t st) {
+  if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, v64.as<double>(),
+                     v32.as<float>());
+  if (!padded) return;
+  if (!d_pad_pos.p) {
+    d_pad_pos.upload(pad_pos.data(), 8 * std::max<int64_t>(1, nnz));
+    hip_check(hipMemsetAsync(v4_64.p, 0, 8 * pad_host.size(), st), "memset");  // padding entries stay 0
+  }
+  hipLaunchKernelGGL(k_scatter_map, dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, d_pad_pos.as<int64_t>(),
+                     (const double*)v64.as<double>(), v4_64.as<double>());
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad_host.size())), dim3(BLOCK), 0, st,
+                     (int64_t)pad_host.size(), v4_64.as<double>(), v4_32.as<float>());
+}
+
 void DevCsr::upload_values(const double* vals, hipStream_t st) {
   v64.upload(vals, 8 * std::max<int64_t>(1, nnz));
   if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));

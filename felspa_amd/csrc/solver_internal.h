@@ -73,6 +73,10 @@ struct DevCsr {
   void upload_pattern(const HostCsr& h, bool with_padded = false);
   // values in the plain layout (host), both precisions and both layouts
   void upload_values(const double* vals, hipStream_t st);
+  // v64 already holds the values (written on the device): the FP32 copy and
+  // the padded layout from it (entry -> padded position kept on the device)
+  void derive_from_v64(hipStream_t st);
+  DevBuf d_pad_pos;
 };
 
 // A CSR matrix stored by row blocks for the blocked SpMV (k_spmv_blk_dot):
@@ -87,6 +91,9 @@ struct BlkCsr {
   std::vector<double> pad;   // host staging of the padded values
   void build(const HostCsr& h, const std::vector<int32_t>& blk_ptr);
   void upload_values(const double* vals, hipStream_t st);
+  // values in the plain layout already in device memory
+  void upload_values_device(const double* d_vals, hipStream_t st);
+  DevBuf d_pos;
 };
 
 struct IluDev {

@@ -218,6 +218,18 @@ __global__ __launch_bounds__(BLOCK) void k_velocity_at_points(int64_t np, const 
 
 }  // namespace
 
+// dst[map[e]] = src[e] / dst[e] = src[map[e]]
+static __global__ __launch_bounds__(BLOCK) void k_values_scatter(int64_t n, const int64_t* __restrict__ map,
+                                                          const double* __restrict__ src, double* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * BLOCK)
+    dst[map[e]] = src[e];
+}
+static __global__ __launch_bounds__(BLOCK) void k_values_gather(int64_t n, const int64_t* __restrict__ map,
+                                                         const double* __restrict__ src, double* __restrict__ dst) {
+  for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * BLOCK)
+    dst[e] = src[map[e]];
+}
+
 struct fs_handle {
   fs_opts o{};
   int64_t nu = 0, np = 0, n = 0, ld = 0;
@@ -227,6 +239,7 @@ struct fs_handle {
   std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp;  // caller entry -> K / Mp position
   std::vector<int64_t> ablk_from_k;                       // A entry -> K position
   std::vector<double> hK, hMp, hA;                        // host value staging
+  DevBuf dmapA, dmapBt, dmapB, dmapK11, dmapMp, dablk;   // the maps above on the device (device refactor)
   IluDev FA, FS;
   DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
   int64_t ldu = 0;
@@ -629,6 +642,42 @@ struct fs_handle {
     for (size_t e = 0; e < mapK11.size(); ++e) hK[mapK11[e]] = K11[e];
     for (size_t e = 0; e < mapMp.size(); ++e) hMp[mapMp[e]] = M[e];
     upload_values();
+  }
+
+  // New values already in device memory (the caller's layouts): scattered
+  // into K and M_p by device copies of the value maps, A alone gathered from
+  // K; the maps go to the device on first use.
+  void set_values_device(const double* A, const double* Bt, const double* B, const double* K11, const double* M) {
+    if (!dmapA.p) {
+      dmapA.upload(mapA.data(), 8 * std::max<size_t>(1, mapA.size()));
+      dmapBt.upload(mapBt.data(), 8 * std::max<size_t>(1, mapBt.size()));
+      dmapB.upload(mapB.data(), 8 * std::max<size_t>(1, mapB.size()));
+      dmapK11.upload(mapK11.data(), 8 * std::max<size_t>(1, mapK11.size()));
+      dmapMp.upload(mapMp.data(), 8 * std::max<size_t>(1, mapMp.size()));
+      dablk.upload(ablk_from_k.data(), 8 * std::max<size_t>(1, ablk_from_k.size()));
+    }
+    auto scatter = [&](const DevBuf& map, size_t n, const double* src, DevBuf& dst) {
+      if (n == 0) return;
+      hipLaunchKernelGGL(k_values_scatter, dim3(grid((int64_t)n)), dim3(BLOCK), 0, st, (int64_t)n,
+                         map.as<int64_t>(), src, dst.as<double>());
+    };
+    scatter(dmapA, mapA.size(), A, K.v64);
+    scatter(dmapBt, mapBt.size(), Bt, K.v64);
+    scatter(dmapB, mapB.size(), B, K.v64);
+    scatter(dmapK11, mapK11.size(), K11, K.v64);
+    scatter(dmapMp, mapMp.size(), M, Mp.v64);
+    K.derive_from_v64(st);
+    const int64_t na = (int64_t)ablk_from_k.size();
+    if (na > 0)
+      hipLaunchKernelGGL(k_values_gather, dim3(grid(na)), dim3(BLOCK), 0, st, na, dablk.as<int64_t>(),
+                         (const double*)K.v64.as<double>(), Ablk.v64.as<double>());
+    Ablk.derive_from_v64(st);
+    Ablb.upload_values_device(Ablk.v64.as<double>(), st);
+    Mp.derive_from_v64(st);
+    hip_check(hipGetLastError(), "device refactor");
+    hip_check(hipStreamSynchronize(st), "sync");
+    cnt.launches += 12;
+    dirty = true;
   }
 
   // network guess into x (solver order); falls back to zero when its
@@ -1230,6 +1279,19 @@ int fs_stokes_refactor(fs_handle* h, const double* A_val, const double* Bt_val, 
   }
   FS_GUARD_BEGIN
   h->set_values(A_val, Bt_val, B_val, K11_val, Mp_val);
+  return FS_OK;
+  FS_GUARD_END
+}
+
+int fs_stokes_refactor_device(fs_handle* h, const double* d_A, const double* d_Bt, const double* d_B,
+                              const double* d_K11, const double* d_Mp) {
+  if (!h || !d_A || !d_Bt || !d_B || !d_K11 || !d_Mp) {
+    fs::set_error("fs_stokes_refactor_device: null argument");
+    return FS_ERR_BAD_INPUT;
+  }
+  FS_GUARD_BEGIN
+  hip_check(hipStreamSynchronize(nullptr), "sync");
+  h->set_values_device(d_A, d_Bt, d_B, d_K11, d_Mp);
   return FS_OK;
   FS_GUARD_END
 }

@@ -136,6 +136,12 @@ class StokesLinearSystem:
         L.check(self._lib.fs_stokes_refactor(self.h, *[a.ctypes.data_as(L.P(L.f64)) for a in arrs]),
                 "fs_stokes_refactor")
 
+    def refactor_device(self, d_ptrs):
+        """New values already on the device: d_ptrs = raw pointers of the A,
+        Bt, B, K11, Mp value arrays (the layouts given at create)."""
+        L.check(self._lib.fs_stokes_refactor_device(self.h, *[C.c_void_p(int(p)) for p in d_ptrs]),
+                "fs_stokes_refactor_device")
+
     def solve(self, soln, rhs, control=None, solution_method=None):
         """StokesLinearSystem::solve(soln, rhs, solver_control) (stokes_ilu.cc:442-468).
 

@@ -78,12 +78,78 @@ class StokesSystem:
         self._lib.fs_system_distribute(self.h, x.ctypes.data_as(L.P(L.f64)))
         return x
 
+    @property
+    def n_cells_total(self):
+        return self.n_cells ** self.dim
+
+    def quadrature_fields(self):
+        """(eta_q, rho_q): the field at the assembly's quadrature points,
+        [n_cells^dim, 3^dim] each (cells x fastest)."""
+        nq = 3 ** self.dim
+        eta = np.zeros((self.n_cells_total, nq))
+        rho = np.zeros((self.n_cells_total, nq))
+        L.check(self._lib.fs_system_quadrature_fields(self.h, eta.ctypes.data_as(L.P(L.f64)),
+                                                      rho.ctypes.data_as(L.P(L.f64))),
+                "fs_system_quadrature_fields")
+        return eta, rho
+
     def block_hint(self, nodes_per_block):
         out = np.zeros(self.n_u, np.int32)
         L.check(self._lib.fs_system_block_hint(self.h, int(nodes_per_block),
                                                 out.ctypes.data_as(L.P(L.i32))),
                 "fs_system_block_hint")
         return out
+
+
+class DeviceAssembler:
+    """fs_assembler: the system's blocks and rhs re-assembled on the GPU on
+    the system's patterns (StokesAssembler::local_assembly +
+    copy_local_to_global). Buffers are torch CUDA tensors."""
+
+    def __init__(self, system):
+        import torch
+        self._lib = L.lib()
+        self.h = C.c_void_p()
+        L.check(self._lib.fs_assembler_create(system.h, C.byref(self.h)), "fs_assembler_create")
+        self.dim, self.n_cells, self.n_u, self.n_p = system.dim, system.n_cells, system.n_u, system.n_p
+        self.n_q = system.n_cells_total * 3 ** system.dim
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.values = {name: torch.zeros(system.blocks[name].nnz, dtype=torch.float64, device=dev)
+                       for name in BLOCK_NAMES}
+        self.rhs = torch.zeros(system.n, dtype=torch.float64, device=dev)
+
+    def sample_field(self):
+        """(eta_q, rho_q) device tensors of the system's own field."""
+        import torch
+        eta = torch.empty(self.n_q, dtype=torch.float64, device=self.rhs.device)
+        rho = torch.empty_like(eta)
+        torch.cuda.synchronize()
+        L.check(self._lib.fs_assembler_sample_field(self.h, C.c_void_p(eta.data_ptr()), C.c_void_p(rho.data_ptr())),
+                "fs_assembler_sample_field")
+        return eta, rho
+
+    def assemble(self, eta_q, rho_q):
+        """Assembles into self.values / self.rhs from device samples; returns
+        the GPU milliseconds of the assembly."""
+        import torch
+        assert eta_q.is_cuda and eta_q.dtype == torch.float64 and eta_q.numel() == self.n_q
+        assert rho_q.is_cuda and rho_q.dtype == torch.float64 and rho_q.numel() == self.n_q
+        eta_q, rho_q = eta_q.contiguous(), rho_q.contiguous()
+        torch.cuda.synchronize()
+        v = self.values
+        L.check(self._lib.fs_assembler_assemble(
+            self.h, C.c_void_p(eta_q.data_ptr()), C.c_void_p(rho_q.data_ptr()),
+            *[C.c_void_p(v[name].data_ptr()) for name in BLOCK_NAMES], C.c_void_p(self.rhs.data_ptr())),
+            "fs_assembler_assemble")
+        return self._lib.fs_assembler_last_ms(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fs_assembler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def save_blocks(path, blocks, rhs=None):

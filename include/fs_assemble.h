@@ -106,6 +106,39 @@ const float* fs_system_log_eta_grid(const fs_system* sys);
  * when the node lattice is cut into cubes of `nodes_per_block` nodes per
  * direction. Written into out[n_u]. */
 int fs_system_block_hint(const fs_system* sys, int32_t nodes_per_block, int32_t* out);
+/* Viscosity and density at the quadrature points the assembly uses
+ * (viscous_flow.implement.h:42-72 evaluated at the points as stokes_ilu.cc:762-772 does):
+ * eta_q, rho_q [n_cells^dim * 3^dim], cell-major (cells x fastest, then the
+ * cell's Gauss points with index (q%3, q/3%3, q/9%3)). */
+int fs_system_quadrature_fields(const fs_system* sys, double* eta_q, double* rho_q);
+
+/* ---------------------------------------------------------------------------
+ * Device assembly (GPU). StokesAssembler::local_assembly (stokes_ilu.cc:739-821)
+ * and copy_local_to_global (stokes_common.implement.h:658-669) with the
+ * constraint condensation of the host generator, on the patterns of an
+ * existing system: values of A, B^T, B, K11, M_p and the rhs are written to
+ * caller device arrays laid out exactly like fs_system_csr's blocks, and are
+ * bit-identical to the host generator's for the same quadrature samples.
+ * Every time step that changes the viscosity only re-runs
+ * fs_assembler_assemble (then fs_stokes_refactor_device and
+ * fs_stokes_solve_device): no host assembly and no host-device copy.
+ * Calls on one assembler are serialised by the caller; each call completes
+ * before it returns. */
+typedef struct fs_assembler fs_assembler;
+/* uploads the patterns, dof maps and constraints of `sys` (which may be
+ * destroyed afterwards) to the current device */
+int fs_assembler_create(const fs_system* sys, fs_assembler** out);
+/* the system's own viscosity / density field at the quadrature points, as
+ * fs_system_quadrature_fields lays them out, into device arrays */
+int fs_assembler_sample_field(fs_assembler* a, double* d_eta_q, double* d_rho_q);
+/* assembles from device samples d_eta_q, d_rho_q [n_cells^dim * 3^dim] into
+ * device value arrays of nnz(A), nnz(Bt), nnz(B), nnz(K11), nnz(Mp) and
+ * d_rhs [n_u + n_p] */
+int fs_assembler_assemble(fs_assembler* a, const double* d_eta_q, const double* d_rho_q, double* d_A,
+                          double* d_Bt, double* d_B, double* d_K11, double* d_Mp, double* d_rhs);
+/* GPU time of the last fs_assembler_assemble (ms, HIP events on its stream) */
+double fs_assembler_last_ms(const fs_assembler* a);
+void fs_assembler_destroy(fs_assembler* a);
 
 #ifdef __cplusplus
 }

@@ -128,6 +128,13 @@ int fs_stokes_create(const fs_csr* A, const fs_csr* Bt, const fs_csr* B, const f
  * stokes_ilu.cc:547-548): numeric factorisation happens in the next solve. */
 int fs_stokes_refactor(fs_handle* h, const double* A_val, const double* Bt_val,
                        const double* B_val, const double* K11_val, const double* Mp_val);
+/* The same with the values already in device memory (FP64, the caller's
+ * layouts given at create), e.g. written by fs_assembler_assemble: device
+ * copies of the value maps (about 8 bytes per entry of K and M_p, allocated
+ * on the first call) scatter them into the solver's layouts; nothing passes
+ * through the host. */
+int fs_stokes_refactor_device(fs_handle* h, const double* d_A, const double* d_Bt,
+                              const double* d_B, const double* d_K11, const double* d_Mp);
 
 /* StokesLinearSystem::solve (stokes_ilu.cc:442-468 -> solve_gmres :509-561):
  * rhs [n_u+n_p] host, x [n_u+n_p] host: guess in, solution out. The caller

@@ -106,3 +106,44 @@ def test_inclusion_viscosity_rule():
 def test_bad_problem_rejected():
     with pytest.raises(L.FsException):
         fa.StokesSystem(fa.problem(4, 4, 0, 0))
+
+
+def test_quadrature_fields_inclusion_and_layers():
+    """The samples the assembly uses: the field at (cell + Gauss point) * h,
+    with Gauss points 0.5 -+ sqrt(3/5)/2 and 0.5 (3-point Gauss-Legendre on
+    [0, 1]), restated with numpy; inclusion and layered fields."""
+    g = np.sqrt(3.0 / 5.0) / 2.0
+    gp = np.array([0.5 - g, 0.5, 0.5 + g])
+    for dim, n in ((2, 7), (3, 4)):
+        S = fa.StokesSystem(fa.problem(dim, n, L.FS_FIELD_INCLUSION, L.FS_BC_FREE_SLIP, eta_matrix=2.0,
+                                 eta_incl=50.0, rho_matrix=1.0, rho_incl=3.0, radius=0.31))
+        eta, rho = S.quadrature_fields()
+        assert eta.shape == (n ** dim, 3 ** dim)
+        cells = np.indices((n,) * dim).reshape(dim, -1)[::-1]          # axis k of cell c, x fastest
+        qs = np.indices((3,) * dim).reshape(dim, -1)[::-1]             # Gauss index k of point q
+        x = (cells[:, :, None] + gp[qs][:, None, :]) / n                # [dim, cell, q]
+        dist = np.sqrt(((x - 0.5) ** 2).sum(0))
+        nrm = np.sqrt((x ** 2).sum(0))
+        want = np.where(dist < 0.31, 50.0, np.where(nrm > 0.31, 2.0, 26.0))
+        assert np.array_equal(eta, want)
+        assert np.array_equal(rho, np.where(dist < 0.31, 3.0, np.where(nrm > 0.31, 1.0, 2.0)))
+        S = fa.StokesSystem(fa.problem(dim, n, L.FS_FIELD_LAYERS, L.FS_BC_FREE_SLIP, eta_matrix=1.0,
+                                 eta_incl=10.0, amp=0.1))
+        eta, _ = S.quadrature_fields()
+        y = 0.5 + 0.1 * np.cos(2 * np.pi * x[0])
+        upper = x[dim - 1] > y
+        near = np.abs(x[dim - 1] - y) < 1e-12
+        assert np.array_equal(eta[~near], np.where(upper, 10.0, 1.0)[~near])
+
+
+def test_pinned_pressure_diagonal_is_mean_free_diagonal():
+    """Constrained pressure rows carry K11 = mean |A_ii| over the free
+    velocity rows (summed by row chunks; equal to the plain mean to rounding)."""
+    S = fa.StokesSystem(fa.problem(2, 12, L.FS_FIELD_INCLUSION, L.FS_BC_TOPBOTTOM_SHEAR_PERIODIC))
+    A, K11 = S.blocks["A"], S.blocks["K11"]
+    free = ~S.constrained[:S.n_u]
+    want = np.abs(A.diagonal()[free]).mean()
+    pc = S.constrained[S.n_u:]
+    assert pc.sum() >= 2  # the pinned dof and the periodic pressure slaves
+    assert np.allclose(K11.diagonal()[pc], want, rtol=1e-14, atol=0)
+    assert np.all(K11.diagonal()[~pc] == 0.0)
