@@ -869,10 +869,12 @@ struct Idx4<2> {
   }
 };
 
-// Blocks with more than DENSE_SPLIT_ABOVE chunks are summed in pieces of
-// DENSE_SPLIT chunks by k_ilu_ext_partials, one workgroup per piece
-constexpr int DENSE_SPLIT_ABOVE = 1536;
-constexpr int DENSE_SPLIT = 1536;
+// Blocks with more external chunks than a piece are summed in pieces by
+// k_ilu_ext_partials, one workgroup per piece: pieces of 1536 chunks, or of
+// 3072 for factors with long rows (more bytes in flight per workgroup and
+// fewer, larger halos)
+constexpr int DENSE_PIECE_SMALL = 1536;
+constexpr int DENSE_PIECE_LARGE = 3072;
 constexpr int DENSE_SPLIT_NT = 512;
 
 // Chunks of external entries per thread loaded with the inverse in the
@@ -923,21 +925,24 @@ __device__ __forceinline__ T chunk_dot(const V4& v, typename Idx4<IB>::type ix, 
 }
 
 // External sums of one piece (chunks [ca, ce) of block blk, ce - ca <=
-// DENSE_SPLIT) of a block with many external entries: the piece's halo (its
+// PIECE) of a block with many external entries: the piece's halo (its
 // entries' distinct columns; their indices point into it) is gathered into
-// LDS, then every row's sum over its chunks inside the piece,
-// in chunk order, goes to part[first + piece rows + row]. One workgroup per
-// piece, all chunk loads in one round.
-template <typename TV, typename T, typename TY, int IB>
+// LDS, every chunk's sum goes to LDS, then G threads per row sum the row's
+// chunks inside the piece (thread g takes chunks g, g + G, ... of the row)
+// and the G partial sums are added in order into part[first + piece rows +
+// row]. One workgroup per piece, all chunk loads in one round. LDS: PIECE
+// chunk sums, the halo, NT partial row sums.
+template <typename TV, typename T, typename TY, int IB, int PIECE>
 __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV> F, const i32x4* __restrict__ pieces,
-                                                                     int pc0, const TY* yin) {
+                                                                     int pc0, int hcap, const TY* yin) {
   typedef TV V4 __attribute__((ext_vector_type(4)));
   typedef typename Idx4<IB>::type IX;
   constexpr int NT = DENSE_SPLIT_NT;
-  constexpr int R = (DENSE_SPLIT + NT - 1) / NT;
+  constexpr int R = (PIECE + NT - 1) / NT;
   extern __shared__ __attribute__((aligned(16))) char dsm[];
   T* part = (T*)dsm;
-  T* hv = part + DENSE_SPLIT;
+  T* hv = part + PIECE;
+  T* red = hv + hcap;
   const int tid = threadIdx.x;
   const i32x4 pc = pieces[2 * (pc0 + blockIdx.x)];      // {block, first chunk, end chunk, piece}
   const i32x4 ph = pieces[2 * (pc0 + blockIdx.x) + 1];  // {the piece's halo, halo columns, 0, 0}
@@ -956,9 +961,12 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
     ix[u] = __builtin_nontemporal_load(isrc + cc);
     vv[u] = __builtin_nontemporal_load(vsrc + cc);
   }
+  // G threads per row (nr <= 256 rows per block)
+  const int G = 4 * nr <= NT ? 4 : 2 * nr <= NT ? 2 : 1;
+  const int row = tid / G, g = tid - row * G;
   int cs = 0, cn = 0;
-  if (tid < nr) {
-    const uint32_t rg = (uint32_t)F.rng[r0 + tid];
+  if (row < nr) {
+    const uint32_t rg = (uint32_t)F.rng[r0 + row];
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
   }
@@ -970,15 +978,23 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
     if (c < ce) part[c - ca] = chunk_dot<T, V4, IB>(vv[u], ix[u], hv);
   }
   __syncthreads();
-  if (tid < nr) {
-    const int2 sp = F.split[blk];
+  if (row < nr) {
     const int lo = cs > ca ? cs : ca, hi = cs + cn < ce ? cs + cn : ce;
     T s0 = 0, s1 = 0;
-    for (int k = lo; k < hi; ++k) {
-      if ((k - cs) & 1) s1 += part[k - ca];
-      else s0 += part[k - ca];
+    int k = lo + g;
+    for (; k + G < hi; k += 2 * G) {
+      s0 += part[k - ca];
+      s1 += part[k + G - ca];
     }
-    ((T*)F.part)[sp.y + (size_t)piece * nr + tid] = s0 + s1;
+    if (k < hi) s0 += part[k - ca];
+    red[g * nr + row] = s0 + s1;
+  }
+  __syncthreads();
+  if (tid < nr) {
+    const int2 sp = F.split[blk];
+    T s = red[tid];
+    for (int q = 1; q < G; ++q) s += red[q * nr + tid];
+    ((T*)F.part)[sp.y + (size_t)piece * nr + tid] = s;
   }
 }
 

@@ -397,6 +397,13 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     uinv64.alloc(8 * std::max<int64_t>(4, inv_total));
     linv32.alloc(4 * std::max<int64_t>(4, inv_total));
     uinv32.alloc(4 * std::max<int64_t>(4, inv_total));
+    // pieces of split blocks: 3072 chunks when rows are long (3D ILU(2): about
+    // 1,500 entries per row), else 1536; FS_ILU_PIECE=1536|3072 overrides
+    piece = F.nnz() > 600 * n ? DENSE_PIECE_LARGE : DENSE_PIECE_SMALL;
+    if (const char* e = std::getenv("FS_ILU_PIECE")) {
+      const int v = std::atoi(e);
+      if (v == DENSE_PIECE_SMALL || v == DENSE_PIECE_LARGE) piece = v;
+    }
     // external entries, L part [0, o_lint) and U part [o_uext, len) of every
     // row, in chunks of 4: every row's list padded to a multiple of 4
     // (padding: position -1 = value 0, column = the row's first external
@@ -433,8 +440,8 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       // the block's halo: its distinct external columns, ascending (nearby
       // lanes gather nearby values); every entry stores its column's index in
       // the halo (1 byte when no halo exceeds 256 columns, else 2); halos
-      // start at multiples of 4. A block of more than DENSE_SPLIT_ABOVE chunks
-      // has a halo per piece of DENSE_SPLIT chunks instead, listed in phl.
+      // start at multiples of 4. A block of more than `piece` chunks has a
+      // halo per piece of `piece` chunks instead, listed in phl.
       // Padding entries repeat the row's first external column. Blocks are
       // independent: halos are formed in parallel, then laid end to end.
       std::vector<uint16_t> loc(std::max<int64_t>(4, n_ent), 0);
@@ -456,9 +463,9 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
             for (int64_t p = p0; p < p1; ++p) cc.push_back(F.ci[base + p]);
             while ((cc.size() - e0) % 4) cc.push_back(cc[e0]);
           }
-          const bool split = nc > DENSE_SPLIT_ABOVE;
-          for (int c = 0; c < (split ? nc : std::min(nc, 1)); c += split ? DENSE_SPLIT : nc) {
-            const size_t ea = 4 * (size_t)c, ez = 4 * (size_t)(split ? std::min(nc, c + DENSE_SPLIT) : nc);
+          const bool split = nc > piece;
+          for (int c = 0; c < (split ? nc : std::min(nc, 1)); c += split ? piece : nc) {
+            const size_t ea = 4 * (size_t)c, ez = 4 * (size_t)(split ? std::min(nc, c + piece) : nc);
             hb.clear();
             for (size_t e2 = ea; e2 < ez; ++e2)
               if (stamp[cc[e2]] != b) {
@@ -492,7 +499,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       phl.assign(nb, {});
       for (int b = 0; b < nb; ++b) {
         std::copy(bh[b].begin(), bh[b].end(), hc.begin() + hoff[b]);
-        const bool split = dsc[4 * b + 3] > DENSE_SPLIT_ABOVE;
+        const bool split = dsc[4 * b + 3] > piece;
         for (size_t k2 = 0; k2 + 1 < bpc[b].size(); k2 += 2) {
           const int32_t off = (int32_t)(hoff[b] + bpc[b][k2]), cnt = bpc[b][k2 + 1];
           if (split) {
@@ -530,7 +537,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         pl);
     ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, xu_phmax, n_xuh,
         pu);
-    // blocks with many external chunks are summed in pieces of DENSE_SPLIT
+    // blocks with many external chunks are summed in pieces of `piece`
     // chunks by workgroups of their own; pieces listed stage by stage
     int64_t part_need = 0;
     auto split = [&](const std::vector<int32_t>& dsc, const std::vector<std::vector<int32_t>>& phl, DevBuf& dsplit,
@@ -542,14 +549,14 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       for (int st = 0; st < sched.n_stages(); ++st) {
         for (int b = sched.stage_ptr[st]; b < sched.stage_ptr[st + 1]; ++b) {
           const int nr = dsc[4 * b + 1], nc = dsc[4 * b + 3];
-          const int S = nc > DENSE_SPLIT_ABOVE ? (nc + DENSE_SPLIT - 1) / DENSE_SPLIT : 1;
+          const int S = nc > piece ? (nc + piece - 1) / piece : 1;
           sp[2 * b] = S;
           sp[2 * b + 1] = (int32_t)off;
           if (S > 1) {
             for (int q = 0; q < S; ++q) {  // {block, first chunk, end chunk, piece}, {halo, halo columns, 0, 0}
               pcs.push_back(b);
-              pcs.push_back(q * DENSE_SPLIT);
-              pcs.push_back(std::min(nc, (q + 1) * DENSE_SPLIT));
+              pcs.push_back(q * piece);
+              pcs.push_back(std::min(nc, (q + 1) * piece));
               pcs.push_back(q);
               pcs.push_back(phl[b][2 * q]);
               pcs.push_back(phl[b][2 * q + 1]);
@@ -832,14 +839,15 @@ static void launch_dense_ib(const IluDev& F, int b0, int b1, const TB* b, const 
   while (F.sched.stage_ptr[s] != b0) ++s;
   if (pp[s + 1] > pp[s]) {
     const int hcap = ((fwd ? F.xl_phmax : F.xu_phmax) + 3) & ~3;
-    const size_t lds = sizeof(T) * ((size_t)DENSE_SPLIT + hcap);
-    auto fn = &k_ilu_ext_partials<TV, T, TY, IB>;
+    const size_t lds = sizeof(T) * ((size_t)F.piece + hcap + DENSE_SPLIT_NT);
+    auto fn = F.piece == DENSE_PIECE_LARGE ? &k_ilu_ext_partials<TV, T, TY, IB, DENSE_PIECE_LARGE>
+                                           : &k_ilu_ext_partials<TV, T, TY, IB, DENSE_PIECE_SMALL>;
     if (lds > dense_wg_lds_limit()) throw FsError(FS_ERR_UNSUPPORTED, "ILU block halo too large");
     if (lds > 64 * 1024)
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                 "hipFuncSetAttribute");
     hipLaunchKernelGGL(fn, dim3(pp[s + 1] - pp[s]), dim3(DENSE_SPLIT_NT), lds, st, dense_view<TV>(F, fwd),
-                       fwd ? F.xlpieces.as<i32x4>() : F.xupieces.as<i32x4>(), pp[s], yin);
+                       fwd ? F.xlpieces.as<i32x4>() : F.xupieces.as<i32x4>(), pp[s], hcap, yin);
   }
   // 128 threads for blocks of up to 64 rows (more workgroups resident per
   // CU), 512 above (larger blocks carry many more external entries: more of

@@ -115,6 +115,7 @@ struct IluDev {
     std::vector<int> first;
     int nrows = 0;
   };
+  int piece = fsk::DENSE_PIECE_SMALL;  // external chunks per piece of a split block
   bool staged = false;
   int64_t ikj_updates = 0;  // numeric work: sum over L entries (i, k) of |U part of row k|
   bool need_v32 = true;      // FP32 CSR copy of the factor (substitution kernels)
