@@ -276,8 +276,9 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
 // column list (its distinct columns, ascending) through 1- or 2-byte indices,
 // rows padded to chunks of 4. One workgroup per block: the chunks (16-byte
 // value vectors + index words) and the column list load in one round, the
-// block's p values are gathered once into LDS, chunk sums go to LDS and
-// every row sums its chunks in order (two partial sums by chunk parity).
+// block's p values are gathered once into LDS, chunk sums go to LDS, G
+// threads per row add the row's chunks (thread g: chunks g, g + G, ...) and
+// the G partial sums are added in order.
 template <int IB>
 struct Idx4;
 template <typename TV, typename T, int IB, int NT>
@@ -314,11 +315,14 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
   // list, all in the first burst: unconditional loads of clamped indices
   // (nr >= 1; the column list holds at least the block's own rows)
   const int tr = tid < nr ? tid : nr - 1;
-  const uint32_t rg = (uint32_t)rng[r0 + tr];
   const T pr = p[r0 + tr];
   const T rr = (rdot ? rdot : p)[r0 + tr];
-  const int cs = tid < nr ? (int)(rg & 0xffffu) : 0;
-  const int cn = tid < nr ? (int)(rg >> 16) : 0;
+  // G threads per row in the row sums
+  const int G = 4 * nr <= NT ? 4 : 2 * nr <= NT ? 2 : 1;
+  const int grow = tid / G, gq = tid - grow * G;
+  const uint32_t rg = (uint32_t)rng[r0 + (grow < nr ? grow : nr - 1)];
+  const int cs = grow < nr ? (int)(rg & 0xffffu) : 0;
+  const int cn = grow < nr ? (int)(rg >> 16) : 0;
   const i32x4* c4 = (const i32x4*)(cols + ci.x);
   const i32x4 cf = __builtin_nontemporal_load(c4 + (4 * tid < ci.y ? tid : 0));
   {
@@ -370,14 +374,22 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
     }
   }
   __syncthreads();
+  __shared__ T red[NT];
+  if (grow < nr) {  // thread gq of a row: its chunks gq, gq + G, ...
+    T s0 = 0, s1 = 0;
+    int k = gq;
+    for (; k + G < cn; k += 2 * G) {
+      s0 += part[cs + k];
+      s1 += part[cs + k + G];
+    }
+    if (k < cn) s0 += part[cs + k];
+    red[gq * nr + grow] = s0 + s1;
+  }
+  __syncthreads();
   double contrib = 0, c2 = 0;
   if (tid < nr) {
-    T s0 = 0, s1 = 0;
-    for (int k = 0; k < cn; ++k) {
-      if (k & 1) s1 += part[cs + k];
-      else s0 += part[cs + k];
-    }
-    const T acc = s0 + s1;
+    T acc = red[tid];
+    for (int g = 1; g < G; ++g) acc += red[g * nr + tid];
     q[r0 + tid] = acc;
     contrib = (double)acc * (double)pr;
     c2 = rdot ? (double)rr * (double)pr : 0.0;
@@ -952,6 +964,18 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
   const int r0 = d.x, nr = d.y;
   const IX* isrc = (const IX*)F.xidx + d.z;
   const V4* vsrc = (const V4*)F.xv + d.z;
+  // the first round of halo columns goes out before the chunks: the halo
+  // gather then waits only for them (loads complete in issue order) and
+  // overlaps the chunk loads (a piece's halo is never empty)
+  constexpr int UH = 2;
+  const int nhu = (hl.y + 3) / 4;
+  const i32x4* h4 = (const i32x4*)(F.hcol + hl.x);
+  i32x4 hc[UH];
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {
+    const int k = u * NT + tid;
+    hc[u] = __builtin_nontemporal_load(h4 + (k < nhu ? k : 0));
+  }
   IX ix[R];
   V4 vv[R];
 #pragma unroll
@@ -964,13 +988,31 @@ __global__ __launch_bounds__(DENSE_SPLIT_NT) void k_ilu_ext_partials(DenseTri<TV
   // G threads per row (nr <= 256 rows per block)
   const int G = 4 * nr <= NT ? 4 : 2 * nr <= NT ? 2 : 1;
   const int row = tid / G, g = tid - row * G;
+  TY hy[UH][4];
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {  // (column hc[u] of entry 0 stands in past the end)
+    hy[u][0] = yin[hc[u].x];
+    hy[u][1] = yin[hc[u].y];
+    hy[u][2] = yin[hc[u].z];
+    hy[u][3] = yin[hc[u].w];
+  }
   int cs = 0, cn = 0;
   if (row < nr) {
     const uint32_t rg = (uint32_t)F.rng[r0 + row];
     cs = (int)(rg & 0xffffu);
     cn = (int)(rg >> 16);
   }
-  gather_halo<T, TY, NT, 2>(F.hcol + hl.x, hl.y, yin, hv, tid);
+#pragma unroll
+  for (int u = 0; u < UH; ++u) {
+    const int k = u * NT + tid;
+    if (k < nhu) {
+      hv[4 * k] = (T)hy[u][0];
+      hv[4 * k + 1] = (T)hy[u][1];
+      hv[4 * k + 2] = (T)hy[u][2];
+      hv[4 * k + 3] = (T)hy[u][3];
+    }
+  }
+  gather_halo<T, TY, NT, UH>(F.hcol + hl.x, hl.y, yin, hv, tid, UH * NT);
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < R; ++u) {
@@ -1146,9 +1188,17 @@ __global__ __launch_bounds__(NT) void k_ilu_stage_dense(DenseTri<TV> F, const TV
     } else {
       bi = (T)b[r0 + tid];
     }
-    if (!whole) {  // partials in piece order
+    if (!whole) {  // partials in piece order, 8 loads in flight
       const T* P = (const T*)F.part + sp.y + tid;
-      for (int q = 0; q < sp.x; ++q) bi -= P[(size_t)q * nr];
+      int q = 0;
+      for (; q + 8 <= sp.x; q += 8) {
+        T a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = P[(size_t)(q + u) * nr];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) bi -= a[u];
+      }
+      for (; q < sp.x; ++q) bi -= P[(size_t)q * nr];
     }
   }
   // the first round's halo values (column 0 stands in past the end), in
