@@ -649,7 +649,7 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     if (b1 <= b0) continue;
     // LDS sized by the stage's longest row; threads x loads per thread cover
     // the longest U part its L columns can name (one wave up to 256 entries,
-    // four up to 1024, eight waves of 8 loads each beyond)
+    // four up to 1024, sixteen waves of 2 loads each beyond)
     const int ml = stage_max_len[s];
     int hs = 0, win = 0;
     const size_t lds = ilu_numeric_plan(ml, stage_max_nl[s], dense_wg_lds_limit(), hs, win);
@@ -666,17 +666,20 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
   }
     if (um <= 256) FS_NUM(64, 4)
     else if (um <= 1024) FS_NUM(256, 4)
-    else FS_NUM(512, 8)
+    else FS_NUM(1024, 2)
 #undef FS_NUM
     cnt.launches++;
     if (staged && s + 1 < sched.n_stages()) {
-      // the finished stage's columns into every later row
+      // the finished stage's columns into every later row. The update is
+      // latency-bound and its LDS allows two (long rows) to four workgroups
+      // per CU, so the workgroups are wide: 1024 threads x 1 entry per
+      // chunk for rows above 1024 entries, 256 x 2 below (at 48^3, ILU(2):
+      // 256 x 8 took 11.0 s, 256 x 4 10.6 s, 128 x 8 15.8 s, 512 x 2 8.0 s,
+      // 1024 x 1 7.2 s per factorisation)
       const int32_t ca = sched.blk_ptr[b0], cb = sched.blk_ptr[b1];
       for (const NumBin& nb : bins) {
         const int cnt_rows = nb.nrows - nb.first[s];
         if (cnt_rows <= 0) continue;
-        // one wave per row where several rows share a CU's LDS, four where
-        // a row's LDS leaves room for one or two
         const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs, nb.G);
         const int32_t* rl = nb.rows.as<int32_t>() + nb.first[s];
 #define FS_UPD(NT, M, G)                                                                                       \
@@ -688,9 +691,9 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, cnt_rows, ca, cb, nb.mcap, nb.hs);        \
   }
         if (nb.mcap <= 1024) {
-          if (nb.G == 3) FS_UPD(64, 8, 3) else if (nb.G == 2) FS_UPD(64, 8, 2) else FS_UPD(64, 8, 1)
+          if (nb.G == 3) FS_UPD(256, 2, 3) else if (nb.G == 2) FS_UPD(256, 2, 2) else FS_UPD(256, 2, 1)
         } else {
-          if (nb.G == 3) FS_UPD(256, 8, 3) else if (nb.G == 2) FS_UPD(256, 8, 2) else FS_UPD(256, 8, 1)
+          if (nb.G == 3) FS_UPD(1024, 1, 3) else if (nb.G == 2) FS_UPD(1024, 1, 2) else FS_UPD(1024, 1, 1)
         }
 #undef FS_UPD
         cnt.launches++;
