@@ -71,7 +71,7 @@ class FsSystemInfo(C.Structure):
 EXPORTS = [
     "fs_opts_default", "fs_stokes_create", "fs_stokes_refactor", "fs_stokes_refactor_device", "fs_stokes_solve",
     "fs_stokes_solve_device", "fs_stokes_vmult", "fs_stokes_precond_vmult",
-    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_precision", "fs_stokes_set_dof_lattice", "fs_stokes_velocity_lattice", "fs_stokes_velocity_at_points", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
+    "fs_stokes_set_guess_field", "fs_stokes_set_guess_mode", "fs_stokes_set_krylov", "fs_stokes_set_precision", "fs_stokes_set_dof_lattice", "fs_stokes_velocity_lattice", "fs_stokes_velocity_at_points", "fs_stokes_set_method", "fs_stokes_get_ordering", "fs_stokes_get_factor_info", "fs_stokes_get_schedule", "fs_stokes_time_kernel", "fs_last_error", "fs_stokes_destroy",
     "fs_ilu_create", "fs_ilu_factorize", "fs_ilu_vmult", "fs_ilu_vmult_noalloc", "fs_ilu_Tvmult", "fs_ilu_get_factors",
     "fs_ilu_destroy", "fs_ilu_symbolic", "fs_block_color_ordering", "fs_free", "fs_matrix_create", "fs_matrix_vmult",
     "fs_matrix_destroy", "fs_set_device_memory_cap",
@@ -112,6 +112,7 @@ def lib():
         "fs_stokes_precond_vmult": (i32, [V, P(f64), P(f64), i32, P(FsStats)]),
         "fs_stokes_set_guess_field": (i32, [V, V, P(f32), i32, P(i64), P(C.c_int8), P(i64)]),
         "fs_stokes_set_guess_mode": (i32, [V, i32]),
+        "fs_stokes_set_krylov": (i32, [V, i32, i32, f64, i32, f64, i32]),
         "fs_stokes_set_precision": (i32, [V, i32]),
         "fs_stokes_set_dof_lattice": (i32, [V, i32, i32, P(i64), P(C.c_int8)]),
         "fs_stokes_velocity_lattice": (i32, [V, V, V]),

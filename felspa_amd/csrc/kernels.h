@@ -11,7 +11,8 @@ namespace fsk {
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;      // threads per workgroup for streaming kernels
 constexpr int RED_BLOCKS = 512; // partial-sum slots of the two-stage reductions
-constexpr int MAXM = 64;        // max vectors in one multi-dot (restart + 1)
+constexpr int MAXM = 256;       // max vectors in one multi-dot (restart + 1)
+constexpr int MDOT_CHUNK = 32;  // vectors per workgroup row of a multi-dot
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -455,34 +456,39 @@ static __global__ void k_reduce_final(int nb, const double* __restrict__ partial
   if (threadIdx.x == 0) out[0] = mode == 1 ? sqrt(s) : s;
 }
 
-// multi-dot: partial[b*MAXM + k] = sum_i V[k*ld + i] * w[i], k < m
+// multi-dot: partial[b*MAXM + k] = sum_i V[k*ld + i] * w[i], k < m. The
+// vectors are taken MDOT_CHUNK at a time, one chunk per blockIdx.y, so each
+// thread keeps MDOT_CHUNK sums in registers whatever the basis size.
 template <typename TW>
 __global__ __launch_bounds__(BLOCK) void k_mdot_partial(int64_t n, int m, const double* __restrict__ V,
                                                         int64_t ld, const TW* __restrict__ w,
                                                         double* __restrict__ partial) {
-  __shared__ double sh[MAXM][BLOCK / WAVE];
-  double acc[MAXM];
+  __shared__ double sh[MDOT_CHUNK][BLOCK / WAVE];
+  const int k0 = blockIdx.y * MDOT_CHUNK;
+  const int mc = min(MDOT_CHUNK, m - k0);
+  const double* Vc = V + (int64_t)k0 * ld;
+  double acc[MDOT_CHUNK];
 #pragma unroll
-  for (int k = 0; k < MAXM; ++k) acc[k] = 0;
+  for (int k = 0; k < MDOT_CHUNK; ++k) acc[k] = 0;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
     const double wi = (double)w[i];
 #pragma unroll
-    for (int k = 0; k < MAXM; ++k)
-      if (k < m) acc[k] += V[(int64_t)k * ld + i] * wi;
+    for (int k = 0; k < MDOT_CHUNK; ++k)
+      if (k < mc) acc[k] += Vc[(int64_t)k * ld + i] * wi;
   }
   const int wv = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
 #pragma unroll
-  for (int k = 0; k < MAXM; ++k) {
-    if (k < m) {
+  for (int k = 0; k < MDOT_CHUNK; ++k) {
+    if (k < mc) {
       double v = group_reduce<WAVE>(acc[k]);
       if (l == 0) sh[k][wv] = v;
     }
   }
   __syncthreads();
-  if (threadIdx.x < (unsigned)m) {
+  if (threadIdx.x < (unsigned)mc) {
     double s = 0;
     for (int i = 0; i < BLOCK / WAVE; ++i) s += sh[threadIdx.x][i];
-    partial[(int64_t)blockIdx.x * MAXM + threadIdx.x] = s;
+    partial[(int64_t)blockIdx.x * MAXM + k0 + threadIdx.x] = s;
   }
 }
 
@@ -498,21 +504,28 @@ static __global__ void k_mdot_final(int nb, int m, const double* __restrict__ pa
   if (threadIdx.x == 0) out[k] = accumulate ? out[k] + s : s;
 }
 
-// w -= sum_k h[k] V_k ;   optionally partial[b] = sum_i w_i^2 (for the norm)
+// w -= sum_k h[k] V_k ;   optionally partial[b] = sum_i w_i^2 (for the norm).
+// The coefficients sit in LDS; the sum runs over k in order for every row.
 template <typename TW>
 __global__ __launch_bounds__(BLOCK) void k_gs_update(int64_t n, int m, const double* __restrict__ V,
                                                      int64_t ld, const double* __restrict__ h,
                                                      TW* __restrict__ w, double* __restrict__ partial) {
   __shared__ double sh[BLOCK / WAVE];
-  double hk[MAXM];
-#pragma unroll
-  for (int k = 0; k < MAXM; ++k) hk[k] = k < m ? h[k] : 0.0;
+  __shared__ double hk[MAXM];
+  for (int k = threadIdx.x; k < m; k += BLOCK) hk[k] = h[k];
+  __syncthreads();
   double s = 0;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
     double wi = (double)w[i];
+    int k = 0;
+    for (; k + 8 <= m; k += 8) {
+      double v[8];
 #pragma unroll
-    for (int k = 0; k < MAXM; ++k)
-      if (k < m) wi -= hk[k] * V[(int64_t)k * ld + i];
+      for (int u = 0; u < 8; ++u) v[u] = V[(int64_t)(k + u) * ld + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) wi -= hk[k + u] * v[u];
+    }
+    for (; k < m; ++k) wi -= hk[k] * V[(int64_t)k * ld + i];
     w[i] = (TW)wi;
     s += wi * wi;
   }
@@ -533,18 +546,24 @@ __global__ __launch_bounds__(BLOCK) void k_scale_by_inv(int64_t n, const TS* __r
     dst[i] = (double)src[i] * f;
 }
 
-// x += sum_k y[k] Z_k
+// x += sum_k y[k] Z_k (coefficients in LDS, the sum in k order per row)
 template <typename TZ>
 __global__ __launch_bounds__(BLOCK) void k_update_x(int64_t n, int m, const TZ* __restrict__ Z, int64_t ld,
                                                     const double* __restrict__ y, double* __restrict__ x) {
-  double yk[MAXM];
-#pragma unroll
-  for (int k = 0; k < MAXM; ++k) yk[k] = k < m ? y[k] : 0.0;
+  __shared__ double yk[MAXM];
+  for (int k = threadIdx.x; k < m; k += BLOCK) yk[k] = y[k];
+  __syncthreads();
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
     double s = x[i];
+    int k = 0;
+    for (; k + 8 <= m; k += 8) {
+      double v[8];
 #pragma unroll
-    for (int k = 0; k < MAXM; ++k)
-      if (k < m) s += yk[k] * (double)Z[(int64_t)k * ld + i];
+      for (int u = 0; u < 8; ++u) v[u] = (double)Z[(int64_t)(k + u) * ld + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += yk[k + u] * v[u];
+    }
+    for (; k < m; ++k) s += yk[k] * (double)Z[(int64_t)k * ld + i];
     x[i] = s;
   }
 }

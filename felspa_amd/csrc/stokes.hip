@@ -243,6 +243,7 @@ struct fs_handle {
   IluDev FA, FS;
   DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
   int64_t ldu = 0;
+  int m_alloc = 0;          // Krylov basis vectors allocated (largest restart a solve may use)
   double* h_pin = nullptr;  // pinned scalars
   bool dirty = true;
   Counters cnt;
@@ -491,11 +492,12 @@ struct fs_handle {
     const int m = j + 1;
     const int g = std::min<int>(RED_BLOCKS, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
     double* part = dptr<double>(partial);
-    hipLaunchKernelGGL((k_mdot_partial<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
+    const dim3 gd(g, (m + MDOT_CHUNK - 1) / MDOT_CHUNK);
+    hipLaunchKernelGGL((k_mdot_partial<double>), gd, dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
     hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(BLOCK), 0, st, g, m, part, S(SC_H1), 0);
     hipLaunchKernelGGL((k_gs_update<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, S(SC_H1), w,
                        (double*)nullptr);
-    hipLaunchKernelGGL((k_mdot_partial<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
+    hipLaunchKernelGGL((k_mdot_partial<double>), gd, dim3(BLOCK), 0, st, n, m, Vb, ld, w, part);
     hipLaunchKernelGGL(k_mdot_final, dim3(m), dim3(BLOCK), 0, st, g, m, part, S(SC_H2), 0);
     hipLaunchKernelGGL((k_gs_update<double>), dim3(g), dim3(BLOCK), 0, st, n, m, Vb, ld, S(SC_H2), w, part);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(BLOCK), 0, st, g, part, S(SC_NRM), 1);
@@ -1176,7 +1178,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   if (opts) h.o = *opts;
   else fs_opts_default(&h.o);
   auto& o = h.o;
-  if (o.restart < 0 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [0, 63]");
+  if (o.restart < 0 || o.restart > MAXM - 1) throw FsError(FS_ERR_BAD_INPUT, "restart must be in [0, 255]");
   if (o.precision < FS_PREC_FP64 || o.precision > FS_PREC_MIXED_FP64_VECTORS)
     throw FsError(FS_ERR_BAD_INPUT, "unknown precision");
   if (o.ilu_k_A < 0 || o.ilu_k_S < 0 || o.ilu_k_A > 20 || o.ilu_k_S > 20)
@@ -1192,6 +1194,15 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
       hK11.n_rows != h.np || hK11.n_cols != h.np || hM.n_cols != h.np)
     throw FsError(FS_ERR_BAD_INPUT, "block shapes do not match");
   if (h.n >= INT32_MAX) throw FsError(FS_ERR_BAD_INPUT, "system too large");
+  // phase times of the set-up (printed with o.verbose)
+  auto t_last = std::chrono::steady_clock::now();
+  auto phase = [&](const char* what) {
+    const auto t = std::chrono::steady_clock::now();
+    if (o.verbose)
+      std::fprintf(stderr, "fs_stokes_create: %-34s %8.2f s\n", what,
+                   std::chrono::duration<double>(t - t_last).count());
+    t_last = t;
+  };
   // orderings
   const int bmax = 256;
   HostPattern pA = pattern_of(hA), pM = pattern_of(hM);
@@ -1205,15 +1216,19 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
     sS = merge_stage_blocks(schedule_block_color(pM, o.ilu_k_S, 1, nullptr, bmax), 64);
   }
   // permuted patterns for the symbolic phase
+  phase("orderings (block colouring)");
   HostCsr Anew = permute_square(hA, sA.perm, sA.iperm, nullptr);
   std::vector<int64_t> mapM;
   HostCsr Mnew = permute_square(hM, sS.perm, sS.iperm, &mapM);
+  phase("permuted patterns");
   h.FA.keep_pattern = false;
   h.FA.build(pattern_of(Anew), sA, o.ilu_k_A, 8);
+  phase("ILU(k) of A: symbolic + device layout");
   h.FS.dense = false;  // M_p's factor is applied inside the fused inner CG only
   h.FS.build(pattern_of(Mnew), sS, o.ilu_k_S, 8);
   // K in solver order + value maps from the caller's layout
   build_K(h, A, Bt, B, K11);
+  phase("ILU of M_p, K in solver order");
   h.Mp.upload_pattern(Mnew);
   // mapM is indexed by the sorted-layout entries of hM; convert to caller entries
   {
@@ -1226,6 +1241,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.ld = (h.n + 63) & ~int64_t(63);
   // (the method can be switched later: room for either basis)
   const int m = std::max(h.restart_of(FS_METHOD_GMRES_REF), h.restart_of(FS_METHOD_FGMRES));
+  h.m_alloc = m;
   h.b.alloc(8 * h.ld);
   h.x.alloc(8 * h.ld);
   h.r.alloc(8 * h.ld);
@@ -1255,6 +1271,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   hip_check(hipStreamCreateWithFlags(&h.st, hipStreamNonBlocking), "stream");
   hip_check(hipHostMalloc((void**)&h.h_pin, 64, hipHostMallocDefault), "hipHostMalloc");
   h.set_values(A->val, Bt->val, B->val, K11->val, Mp->val);
+  phase("device vectors, values");
 }
 
 extern "C" {
@@ -1579,6 +1596,29 @@ int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess) {
     return FS_ERR_BAD_INPUT;
   }
   h->o.guess = guess;
+  return FS_OK;
+}
+
+int fs_stokes_set_krylov(fs_handle* h, int32_t restart, int32_t inner_a_iters, double inner_cg_tol,
+                         int32_t inner_cg_max, double inner_cg_tol_mixed, int32_t inner_cg_max_mixed) {
+  if (!h) { fs::set_error("fs_stokes_set_krylov: null handle"); return FS_ERR_BAD_INPUT; }
+  fs_opts o = h->o;
+  if (restart >= 0) o.restart = restart;
+  if (inner_a_iters >= 0) o.inner_a_iters = inner_a_iters;
+  if (inner_cg_tol >= 0) o.inner_cg_tol = inner_cg_tol;
+  if (inner_cg_max >= 0) o.inner_cg_max = inner_cg_max;
+  if (inner_cg_tol_mixed >= 0) o.inner_cg_tol_mixed = inner_cg_tol_mixed;
+  if (inner_cg_max_mixed >= 0) o.inner_cg_max_mixed = inner_cg_max_mixed;
+  const int m_ref = o.restart > 0 ? o.restart : 23, m_fg = o.restart > 0 ? o.restart : 60;
+  if (std::max(m_ref, m_fg) > h->m_alloc) {
+    fs::set_error("fs_stokes_set_krylov: restart exceeds the Krylov basis allocated at fs_stokes_create");
+    return FS_ERR_BAD_INPUT;
+  }
+  if (!(o.inner_cg_tol_mixed > 0) || !(o.inner_cg_tol > 0)) {
+    fs::set_error("fs_stokes_set_krylov: inner CG tolerances must be positive");
+    return FS_ERR_BAD_INPUT;
+  }
+  h->o = o;
   return FS_OK;
 }
 

@@ -247,6 +247,19 @@ class StokesLinearSystem:
         L.check(self._lib.fs_stokes_set_guess_mode(self.h, int(guess)), "fs_stokes_set_guess_mode")
         self.opts.guess = guess
 
+    def set_krylov(self, restart=-1, inner_a_iters=-1, inner_cg_tol=-1.0, inner_cg_max=-1,
+                   inner_cg_tol_mixed=-1.0, inner_cg_max_mixed=-1):
+        """Krylov parameters between solves (negative: unchanged); restart is
+        bounded by the basis allocated when the handle was made."""
+        L.check(self._lib.fs_stokes_set_krylov(self.h, int(restart), int(inner_a_iters), float(inner_cg_tol),
+                                               int(inner_cg_max), float(inner_cg_tol_mixed),
+                                               int(inner_cg_max_mixed)), "fs_stokes_set_krylov")
+        for k, v in (("restart", restart), ("inner_a_iters", inner_a_iters), ("inner_cg_tol", inner_cg_tol),
+                     ("inner_cg_max", inner_cg_max), ("inner_cg_tol_mixed", inner_cg_tol_mixed),
+                     ("inner_cg_max_mixed", inner_cg_max_mixed)):
+            if v >= 0:
+                setattr(self.opts, k, v)
+
     def factor_info(self, which=0):
         """ILU factor of A (0) or M_p (1): dict n, nnz, ikj_updates, n_stages."""
         n, nnz, upd, ns = L.i64(), L.i64(), L.i64(), L.i32()

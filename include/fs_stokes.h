@@ -73,7 +73,7 @@ typedef struct fs_opts {
   int32_t ordering;      /* FS_ORDER_* */
   int32_t ilu_k_A;       /* Control::set_level_of_fill A (stokes_ilu.cc:139-147), default 1 */
   int32_t ilu_k_S;       /* ... S, default 0 */
-  int32_t restart;       /* Krylov basis size; 0 (default): the method's own, 23 for
+  int32_t restart;       /* Krylov basis size, at most 255; 0 (default): the method's own, 23 for
                             FS_METHOD_GMRES_REF (deal.II max_n_tmp_vectors 25 - 2,
                             stokes_ilu.cc:538) and 60 for FS_METHOD_FGMRES */
   int32_t max_outer;     /* 0: n_u + n_p (stokes_ilu.cc:534) */
@@ -204,6 +204,16 @@ int fs_stokes_set_precision(fs_handle* h, int32_t precision);
 /* Switch the guess mode (fs_opts.guess) of an existing handle between solves,
  * e.g. to compare FS_GUESS_LEARNED with FS_GUESS_ZERO on the same matrices. */
 int fs_stokes_set_guess_mode(fs_handle* h, int32_t guess);
+
+/* Change the Krylov parameters of an existing handle between solves (the
+ * reference fixes them when it builds its solver objects: SolverGMRES's
+ * max_n_tmp_vectors, stokes_ilu.cc:538, and InverseMatrixCG's control,
+ * linear_operators.implement.h:76): restart (at most the basis allocated at
+ * fs_stokes_create, i.e. the restart given there or the methods' defaults),
+ * inner_a_iters, and the inner CG's tolerance and step cap in FP64 and in
+ * FS_PREC_MIXED. A negative value keeps the current setting. */
+int fs_stokes_set_krylov(fs_handle* h, int32_t restart, int32_t inner_a_iters, double inner_cg_tol,
+                         int32_t inner_cg_max, double inner_cg_tol_mixed, int32_t inner_cg_max_mixed);
 
 /* Row permutation used by FS_ORDER_BLOCK_COLOR for the A (which=0) and
  * M_p (which=1) blocks: perm[new] = old. */
