@@ -156,9 +156,9 @@ def main():
     ap.add_argument("--config", default="C5")
     ap.add_argument("--sample", type=int, default=None, help="seeded field (default: held out from training)")
     ap.add_argument("--n-cells", type=int, default=None)
-    ap.add_argument("--inner-a", type=int, default=None, help="inner ILU-PCG steps (default 15 in 3D, 60 in 2D)")
-    ap.add_argument("--restart", type=int, default=60)
-    ap.add_argument("--hint", type=int, default=None, help="lattice nodes per block edge (default 5 in 2D, 3 in 3D)")
+    ap.add_argument("--inner-a", type=int, default=None, help="inner ILU-PCG steps (default: the configuration's)")
+    ap.add_argument("--restart", type=int, default=None, help="FGMRES basis (default: the configuration's)")
+    ap.add_argument("--hint", type=int, default=None, help="lattice nodes per block edge (default: the configuration's)")
     ap.add_argument("--ilu-k", type=int, default=None, help="override the configuration's ILU level on A")
     ap.add_argument("--weights", default=None, help="network weights (default: the committed ones for the dimension)")
     ap.add_argument("--no-learned", action="store_true", help="headline from the zero guess")
@@ -193,15 +193,16 @@ def main():
     t_gen = time.time() - t0
     log("assembled %s sample %d: n_u=%d n_p=%d nnz(A)=%d in %.1fs" % (cfg, sample, S.n_u, S.n_p,
                                                                        S.blocks["A"].nnz, t_gen))
-    hint_nodes = args.hint or (5 if S.dim == 2 else 3)
-    inner_a = args.inner_a or (60 if S.dim == 2 else 15)
+    hint_nodes = args.hint or settings["hint_nodes"]
+    inner_a = args.inner_a if args.inner_a is not None else settings["inner_a"]
+    restart = args.restart or settings["restart"]
     t0 = time.time()
     # one handle: the mixed and the plain FP64 solves share the symbolic phase
     # and the factor (the precision is switched between solves); the FP64
     # method's inner CG on M_p stops at the same relative 1e-2 / 50 steps
     lin = fa.StokesLinearSystem(S.blocks, A_lof=kA, S_lof=kS, precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
                                 block_hint=S.block_hint(hint_nodes), constraints=S.constrained,
-                                restart=args.restart, inner_a_iters=inner_a, max_outer=3000,
+                                restart=restart, inner_a_iters=inner_a, max_outer=3000,
                                 inner_cg_tol=1e-2, inner_cg_max=50)
     t_create = time.time() - t0
     log("handle: orderings + symbolic ILU(%d) + upload %.1fs" % (kA, t_create))
@@ -360,7 +361,7 @@ def main():
         "data": "synthetic (generated Q2-Q1 system, seeded field)",
         "config": {"workload": WORKLOADS.get(cfg, cfg), "config_id": cfg, "sample": sample,
                    "n_u": S.n_u, "n_p": S.n_p, "nnz_A": int(S.blocks["A"].nnz), "ilu_k_A": kA, "ilu_k_S": kS,
-                   "restart": args.restart, "inner_a_pcg": inner_a, "block_nodes": hint_nodes,
+                   "restart": restart, "inner_a_pcg": inner_a, "block_nodes": hint_nodes,
                    "guess": "learned" if model is not None else "zero", "learned_guess": model is not None,
                    "weights": os.path.relpath(weights, ROOT) if model is not None else None},
         "outer_iters": st_head["outer_iters"], "inner_cg_iters": st_head["inner_iters"],

@@ -1,5 +1,12 @@
 """The five Stokes workloads (matrix/field/BC parameters and solver settings).
 
+Solver settings per workload: ILU levels, lattice nodes per ILU block edge,
+and the flexible method's Krylov basis (restart) and inner ILU-PCG steps on A
+per outer iteration. The last two are the measured optimum on one MI355X
+(mixed precision, FGMRES restart {60, 250} x inner steps 1..60): 2D needs
+15-30 inner steps (C2 stagnates with 8), 3D is fastest with 1-3, where the
+outer iterations grow less than the inner work shrinks.
+
 C1  2D inclusion, 64x64 Q2-Q1, shear top/bottom + periodic x, ILU(0) (CPU-runnable)
 C2  2D Rayleigh-Taylor, 256x256, free slip, gravity, ILU(1)
 C3  3D porphyroclast, 32^3, linear simple shear, ILU(1)
@@ -12,21 +19,23 @@ from .system import problem
 
 CONFIGS = {
     "C1": dict(dim=2, n_cells=64, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_TOPBOTTOM_SHEAR_PERIODIC,
-               eta_matrix=1.0, eta_incl=1e3, radius=0.1, ilu_k_A=0, ilu_k_S=0, hint_nodes=5),
+               eta_matrix=1.0, eta_incl=1e3, radius=0.1, ilu_k_A=0, ilu_k_S=0, hint_nodes=5, restart=250,
+               inner_a=30),
     "C2": dict(dim=2, n_cells=256, field=L.FS_FIELD_LAYERS, bc=L.FS_BC_FREE_SLIP,
                eta_matrix=1.0, eta_incl=100.0, rho_matrix=1.0, rho_incl=1.1, amp=0.02,
-               gravity=1.0, ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+               gravity=1.0, ilu_k_A=1, ilu_k_S=0, hint_nodes=5, restart=250, inner_a=30),
     "C3": dict(dim=3, n_cells=32, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_LINEAR_SHEAR,
-               eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=3),
+               eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=3, restart=250,
+               inner_a=3),
     "C4": dict(dim=2, n_cells=128, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 8), axes=(0.05, 0.2), logc=(0.0, 3.0),
-               ilu_k_A=1, ilu_k_S=0, hint_nodes=5),
+               ilu_k_A=1, ilu_k_S=0, hint_nodes=5, restart=250, inner_a=20),
     "C5": dict(dim=3, n_cells=48, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 4), axes=(0.08, 0.25), logc=(1.0, 3.0),
-               ilu_k_A=2, ilu_k_S=0, hint_nodes=3),
+               ilu_k_A=2, ilu_k_S=0, hint_nodes=3, restart=250, inner_a=3),
 }
 
-_SOLVER_KEYS = ("ilu_k_A", "ilu_k_S", "hint_nodes")
+_SOLVER_KEYS = ("ilu_k_A", "ilu_k_S", "hint_nodes", "restart", "inner_a")
 
 
 def make_problem(name, n_cells=None, sample=0):

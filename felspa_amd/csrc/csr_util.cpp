@@ -1,4 +1,5 @@
 // Host CSR utilities and the per-thread error string behind fs_last_error.
+#include <chrono>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +18,18 @@ void set_error(const char* fmt, ...) {
 }
 
 const char* get_error() { return g_err; }
+
+int g_verbose = 0;
+static std::chrono::steady_clock::time_point g_phase_t = std::chrono::steady_clock::now();
+
+void phase_reset() { g_phase_t = std::chrono::steady_clock::now(); }
+
+void phase_mark(const char* what) {
+  const auto t = std::chrono::steady_clock::now();
+  if (g_verbose)
+    std::fprintf(stderr, "  set-up: %-44s %8.3f s\n", what, std::chrono::duration<double>(t - g_phase_t).count());
+  g_phase_t = t;
+}
 
 HostCsr transpose(const HostCsr& a) {
   HostCsr t;

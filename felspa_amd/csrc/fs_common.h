@@ -29,6 +29,12 @@ struct HostPattern {
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 const char* get_error();
 
+// Set-up phase times: with g_verbose set, phase_mark prints the time since
+// the previous mark (or since phase_reset) to stderr.
+extern int g_verbose;
+void phase_reset();
+void phase_mark(const char* what);
+
 // transpose (pattern + values)
 HostCsr transpose(const HostCsr& a);
 

@@ -1660,10 +1660,12 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ a, i
 
 // LDS of one stage-update workgroup: the G rows' working values from the
 // first updated column on (and a dump slot each), a batch of 64 L columns'
-// U spans and pivots, the columns and the hash map
+// U spans, pivots, first two U values, node-triple flags and steps, the
+// columns and the hash map
 constexpr int UPD_BATCH = 64;
 __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) {
-  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 4) + (size_t)mcap * 4 + (size_t)hs * 4;
+  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 8 + 8 + 4 + 4 + 4) + (size_t)mcap * 4 +
+         (size_t)hs * 4;
 }
 
 // One stage update: the row groups starting at rows[0 .. nrows) (G
@@ -1673,23 +1675,35 @@ __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) 
 // rows of the finished stage. One workgroup of NT threads per group,
 // workgroups XCD-contiguous so that neighbouring rows, which name the same
 // finished rows, share an L2. The rows from their first such column on go to
-// LDS (values per row, the shared columns and hash map); then for the L
-// columns in order, in batches of 64 whose U spans and pivots are read first:
-// each row's m = w_t / u_kk goes to the factor, and the U part of row k (a
-// stream of chunks of NT M entries, loaded three chunks ahead, each entry
-// looked up once for the G rows) updates w_j -= m u_kj where the rows store
-// j. Then the rest goes back to the factor.
+// LDS (values per row, the shared columns and hash map); then the L columns
+// in order, in batches of 64 whose U spans, pivots and first U values are
+// read first. A step is one L column k, or three, k, k+1, k+2, when those
+// rows start a node triple (tri[k]: three consecutive rows of one block with
+// the same columns, so row k's U part starts with k+1, k+2 and row k+1's
+// with k+2, and past k+2 all three U parts name the same columns): the
+// step's multipliers m = w_t / u_kk (for a triple: with the in-node updates
+// of w_{t+1}, w_{t+2} first) go to the factor, then one stream of the U
+// part(s) (chunks of NT M entries, loaded three chunks ahead, each column
+// looked up once for the G rows and the step's 1 or 3 source rows) updates
+// w_j -= m_k u_kj (-= m_k+1 u_k+1,j -= m_k+2 u_k+2,j) where the rows store j.
+// Every w_j receives the same fused multiply-adds in the same order as in
+// the column-by-column IKJ loop. Then the rest goes back to the factor.
 template <int NT, int M, int G>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
-                                                           int nrows, int32_t col_a, int32_t col_b, int mcap,
-                                                           int hs) {
+                                                           const uint8_t* __restrict__ tri, int nrows,
+                                                           int32_t col_a, int32_t col_b, int mcap, int hs) {
   extern __shared__ __attribute__((aligned(16))) char smem[];
   double* w = (double*)smem;                     // [G][mcap + 1]; w[g][mcap]: dump slot
   int64_t* mq = (int64_t*)(w + G * (mcap + 1));  // [UPD_BATCH] first U entry of row k
   double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] pivot u_kk
-  int32_t* mn = (int32_t*)(mu + UPD_BATCH);       // [UPD_BATCH] U entries of row k
-  int32_t* lc = mn + UPD_BATCH;                   // [mcap] columns
+  double* mf = mu + UPD_BATCH;                    // [UPD_BATCH] first U value of row k
+  double* ms = mf + UPD_BATCH;                    // [UPD_BATCH] second U value of row k
+  int32_t* mn = (int32_t*)(ms + UPD_BATCH);       // [UPD_BATCH] U entries of row k
+  int32_t* tf = mn + UPD_BATCH;                   // [UPD_BATCH] slot starts a node triple
+  int32_t* stp = tf + UPD_BATCH;                  // [UPD_BATCH] steps: slot | width << 8
+  int32_t* lc = stp + UPD_BATCH;                  // [mcap] columns
   uint32_t* tab = (uint32_t*)(lc + mcap);         // [hs] hash map
+  __shared__ int n_steps;
   const uint32_t mask = (uint32_t)hs - 1;
   const int lg = __builtin_ctz((uint32_t)hs);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -1723,61 +1737,111 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
   }
   lds_barrier();
   int32_t ja[M], jb[M], jc[M];
-  double va[M], vb[M], vc[M];
+  double va[3][M], vb[3][M], vc[3][M];
   for (int b0 = 0; b0 < nc; b0 += UPD_BATCH) {
     const int nb = min(UPD_BATCH, nc - b0);
     if (tid < nb) {
       const int32_t k = lc[b0 + tid];
       const int64_t kb = F.rp[k];
       const int kd = F.o_diag[k];
-      mq[tid] = kb + kd + 1;
-      mn[tid] = (int)(F.rp[k + 1] - kb - kd - 1);
+      const int nu_k = (int)(F.rp[k + 1] - kb - kd - 1);
+      const int64_t q = kb + kd + 1;
+      mq[tid] = q;
+      mn[tid] = nu_k;
       mu[tid] = F.v[kb + kd];
+      mf[tid] = F.v[nu_k > 0 ? q : kb + kd];
+      ms[tid] = F.v[nu_k > 1 ? q + 1 : kb + kd];
+      tf[tid] = tri[k] != 0 && tid + 2 < nb && lc[b0 + tid + 1] == k + 1 && lc[b0 + tid + 2] == k + 2;
     }
     lds_barrier();
-    // The batch's U parts as one stream of chunks of NT M entries, chunk
-    // (c, off) = entries [off, off + NT M) of column c's U part (a column
-    // with an empty U part still has its chunk: its multipliers). Loads run
-    // three chunks ahead across column boundaries; a barrier precedes the
-    // first chunk of every column (its pivot entries must have all earlier
-    // columns' updates). Positions past the batch load from entry 0.
+    if (tid < 64) {  // wave 0: the batch's steps, in slot order
+      const bool st0 = lane < nb && tf[lane] != 0;
+      const bool mem = (lane >= 1 && lane - 1 < nb && tf[lane - 1] != 0) || (lane >= 2 && lane - 2 < nb && tf[lane - 2] != 0);
+      const bool is_step = lane < nb && !mem;
+      const uint64_t bal = __ballot(is_step);
+      if (is_step) stp[__popcll(bal & ((1ull << lane) - 1ull))] = lane | ((st0 ? 3 : 1) << 8);
+      if (lane == 0) n_steps = __popcll(bal);
+    }
+    lds_barrier();
+    const int ns = n_steps;
+    // The batch's steps as one stream of chunks of NT M entries, chunk
+    // (c, off) = entries [off, off + NT M) of step c's U stream (a step with
+    // an empty U stream still has its chunk: its multipliers). Loads run
+    // three chunks ahead across step boundaries; a barrier precedes the
+    // first chunk of every step (its pivot entries must have all earlier
+    // steps' updates). Positions past the batch load from entry 0; every
+    // chunk issues the loads of a triple, so the loads in flight are the same
+    // on every path.
     constexpr int CHUNK = NT * M;
+    auto slen = [&](int c) {
+      const int sd = stp[c];
+      const int sl = sd & 0xff;
+      return (sd >> 8) == 3 ? mn[sl + 2] : mn[sl];
+    };
     auto adv = [&](int2 p) -> int2 {
-      if (p.x >= nb) return p;
+      if (p.x >= ns) return p;
       p.y += CHUNK;
-      if (p.y >= mn[p.x]) {
+      if (p.y >= slen(p.x)) {
         p.x += 1;
         p.y = 0;
       }
       return p;
     };
-    auto issue = [&](int2 p, int32_t* j, double* v) {
-      const bool ok = p.x < nb;
-      const int s = ok ? p.x : 0;
-      const int64_t q = mq[s];
-      const int n = ok ? mn[s] : 0;
+    auto issue = [&](int2 p, int32_t* j, double (*v)[M]) {
+      const bool ok = p.x < ns;
+      const int sd = ok ? stp[p.x] : 0;
+      const int sl = sd & 0xff;
+      const bool t3 = ok && (sd >> 8) == 3;
+      // the stream's rows: a single column reads row k's U part three times
+      // (the extra values are not used); a triple reads row k past k+1, k+2,
+      // row k+1 past k+2 and row k+2's U part, which name the same columns
+      const int64_t q0 = t3 ? mq[sl] + 2 : mq[sl], q1 = t3 ? mq[sl + 1] + 1 : mq[sl], q2 = t3 ? mq[sl + 2] : mq[sl];
+      const int n = ok ? (t3 ? mn[sl + 2] : mn[sl]) : 0;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
         const int idx = p.y + tid + r * NT;
-        const int64_t a = idx < n ? q + idx : 0;
-        j[r] = F.ci[a];
-        v[r] = F.v[a];
+        const bool in = idx < n;
+        j[r] = F.ci[in ? q2 + idx : 0];
+        v[0][r] = F.v[in ? q0 + idx : 0];
+        v[1][r] = F.v[in ? q1 + idx : 0];
+        v[2][r] = F.v[in ? q2 + idx : 0];
       }
     };
-    double mult[G];
+    double m0[G], m1[G], m2[G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) mult[g] = 0.0;
-    auto process = [&](int2 p, const int32_t* j, const double* v) {
-      if (p.y == 0) {  // a new column: every earlier column's updates are in
+    for (int g = 0; g < G; ++g) m0[g] = m1[g] = m2[g] = 0.0;
+    auto process = [&](int2 p, const int32_t* j, double (*v)[M]) {
+      const int sd = stp[p.x];
+      const int sl = sd & 0xff;
+      const bool t3 = (sd >> 8) == 3;
+      if (p.y == 0) {  // a new step: every earlier step's updates are in
         lds_barrier();
-        const double piv = mu[p.x];
+        const int t = b0 + sl;
+        if (t3) {
+          const double u01 = mf[sl], u02 = ms[sl], u12 = mf[sl + 1];
+          const double p0 = mu[sl], p1 = mu[sl + 1], p2 = mu[sl + 2];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-          mult[g] = w[g * ld + b0 + p.x] / piv;
-          if (tid == g) F.v[bg[g] + b0 + p.x] = mult[g];
+          for (int g = 0; g < G; ++g) {
+            m0[g] = w[g * ld + t] / p0;
+            m1[g] = fma(-m0[g], u01, w[g * ld + t + 1]) / p1;
+            m2[g] = fma(-m1[g], u12, fma(-m0[g], u02, w[g * ld + t + 2])) / p2;
+          }
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            if (tid == 3 * g) F.v[bg[g] + t] = m0[g];
+            if (tid == 3 * g + 1) F.v[bg[g] + t + 1] = m1[g];
+            if (tid == 3 * g + 2) F.v[bg[g] + t + 2] = m2[g];
+          }
+        } else {
+          const double piv = mu[sl];
+#pragma unroll
+          for (int g = 0; g < G; ++g) {
+            m0[g] = w[g * ld + t] / piv;
+            if (tid == g) F.v[bg[g] + t] = m0[g];
+          }
         }
       }
-      const int n = mn[p.x];
+      const int n = t3 ? mn[sl + 2] : mn[sl];
       uint32_t h[M], e[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
@@ -1790,13 +1854,25 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
         const int pp = num_probe(tab, lc, mask, j[r], num_hash(j[r]) & 0xFFFFu, h[r], e[r]);
         pos[r] = p.y + tid + r * NT < n && pp >= 0 ? pp : mcap;
       }
+      if (t3) {
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        double o[M];
+        for (int g = 0; g < G; ++g) {
+          double o[M];
 #pragma unroll
-        for (int r = 0; r < M; ++r) o[r] = w[g * ld + pos[r]];
+          for (int r = 0; r < M; ++r) o[r] = w[g * ld + pos[r]];
 #pragma unroll
-        for (int r = 0; r < M; ++r) w[g * ld + pos[r]] = o[r] - mult[g] * v[r];
+          for (int r = 0; r < M; ++r)
+            w[g * ld + pos[r]] = fma(-m2[g], v[2][r], fma(-m1[g], v[1][r], fma(-m0[g], v[0][r], o[r])));
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          double o[M];
+#pragma unroll
+          for (int r = 0; r < M; ++r) o[r] = w[g * ld + pos[r]];
+#pragma unroll
+          for (int r = 0; r < M; ++r) w[g * ld + pos[r]] = fma(-m0[g], v[0][r], o[r]);
+        }
       }
     };
     int2 p0 = make_int2(0, 0);
@@ -1805,15 +1881,15 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
     issue(p1, jb, vb);
     issue(p2, jc, vc);
     for (;;) {
-      if (p0.x >= nb) break;
+      if (p0.x >= ns) break;
       process(p0, ja, va);
       const int2 p3 = adv(p2);
       issue(p3, ja, va);
-      if (p1.x >= nb) break;
+      if (p1.x >= ns) break;
       process(p1, jb, vb);
       const int2 p4 = adv(p3);
       issue(p4, jb, vb);
-      if (p2.x >= nb) break;
+      if (p2.x >= ns) break;
       process(p2, jc, vc);
       const int2 p5 = adv(p4);
       issue(p5, jc, vc);

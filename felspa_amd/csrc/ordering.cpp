@@ -271,15 +271,22 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched, boo
     std::vector<uint32_t> ucnt((size_t)n * (size_t)k);
     // stage-parallel rows may only read rows of earlier stages or of their
     // own block; a schedule that breaks this is reported, never read through
-    std::vector<int32_t> blk_of, stage_of;
+    // per row: the first row of its block and of its stage (contiguous
+    // ranges), so a left neighbour kk < i is in another block of the same
+    // stage iff stage start <= kk < block start
+    std::vector<int32_t> blk_first, stage_first;
     const bool staged = sched && sched->n == n;
     if (staged) {
-      blk_of.resize(n);
-      stage_of.resize(sched->n_blocks());
-      for (int b = 0; b < sched->n_blocks(); ++b)
-        for (int32_t r = sched->blk_ptr[b]; r < sched->blk_ptr[b + 1]; ++r) blk_of[r] = b;
-      for (int st = 0; st < sched->n_stages(); ++st)
-        for (int b = sched->stage_ptr[st]; b < sched->stage_ptr[st + 1]; ++b) stage_of[b] = st;
+      blk_first.resize(n);
+      stage_first.resize(n);
+      for (int st = 0; st < sched->n_stages(); ++st) {
+        const int32_t s0 = sched->blk_ptr[sched->stage_ptr[st]];
+        for (int b = sched->stage_ptr[st]; b < sched->stage_ptr[st + 1]; ++b)
+          for (int32_t r = sched->blk_ptr[b]; r < sched->blk_ptr[b + 1]; ++r) {
+            blk_first[r] = sched->blk_ptr[b];
+            stage_first[r] = s0;
+          }
+      }
     }
     int conflict = 0;
     auto do_row = [&](int64_t i, std::vector<uint8_t>& mark, std::vector<int32_t>& cols, std::vector<int32_t>& ri,
@@ -297,7 +304,7 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched, boo
         heap.pop();
         const int lk = mark[kk];
         if (lk >= k) continue;
-        if (staged && blk_of[kk] != blk_of[i] && stage_of[blk_of[kk]] >= stage_of[blk_of[i]]) {
+        if (staged && kk >= stage_first[i] && kk < blk_first[i]) {
           bad = 1;  // row kk may still be in the making on another thread
           continue;
         }
@@ -384,19 +391,21 @@ HostPattern symbolic_ilu(const HostPattern& A, int k, const Schedule* sched, boo
 }
 
 bool schedule_is_valid(const HostPattern& F, const Schedule& s) {
-  const int64_t n = F.n;
-  std::vector<int32_t> blk(n), stage(s.n_blocks());
-  for (int b = 0; b < s.n_blocks(); ++b)
-    for (int32_t r = s.blk_ptr[b]; r < s.blk_ptr[b + 1]; ++r) blk[r] = b;
-  for (int st = 0; st < s.n_stages(); ++st)
-    for (int b = s.stage_ptr[st]; b < s.stage_ptr[st + 1]; ++b) stage[b] = st;
+  // stages and blocks are contiguous row ranges: an entry (i, j) joins two
+  // blocks of one stage iff j lies in i's stage range but not in its block
   bool ok = true;
-#pragma omp parallel for schedule(static) reduction(&& : ok)
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t p = F.rp[i]; p < F.rp[i + 1]; ++p) {
-      const int32_t j = F.ci[p];
-      if (blk[i] != blk[j] && stage[blk[i]] == stage[blk[j]]) ok = false;
+#pragma omp parallel for schedule(dynamic, 1) reduction(&& : ok)
+  for (int st = 0; st < s.n_stages(); ++st) {
+    const int32_t s0 = s.blk_ptr[s.stage_ptr[st]], s1 = s.blk_ptr[s.stage_ptr[st + 1]];
+    for (int b = s.stage_ptr[st]; b < s.stage_ptr[st + 1]; ++b) {
+      const int32_t b0 = s.blk_ptr[b], b1 = s.blk_ptr[b + 1];
+      for (int32_t i = b0; i < b1; ++i)
+        for (int64_t p = F.rp[i]; p < F.rp[i + 1]; ++p) {
+          const int32_t j = F.ci[p];
+          if (j >= s0 && j < s1 && (j < b0 || j >= b1)) ok = false;
+        }
     }
+  }
   return ok;
 }
 

@@ -181,17 +181,21 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   HostPattern F = symbolic_ilu(src_pattern_new, k, &sched, &conflict);
   if (conflict || !schedule_is_valid(F, sched))
     throw FsError(FS_ERR_UNSUPPORTED, "ILU fill joins two blocks of one stage; use a larger block size");
+  fs::phase_mark("ILU: symbolic phase + schedule check");
   nnz = F.nnz();
   std::vector<int32_t> ol(n), od(n), ou(n);
   std::vector<int64_t> lc(n + 1, 0), uc(n + 1, 0);
   int64_t sum_lext = 0, sum_uext = 0;
   int64_t cap_need = 0;
-  max_len = 0;
-  // per-row offsets, and per-column counts of in-block entries
+  // per-row offsets, and per-column counts of in-block entries (an in-block
+  // column lies in the row's own block: blocks are independent)
+  int mlen = 0, bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : sum_lext, sum_uext) reduction(max : mlen) \
+    reduction(| : bad)
   for (int b = 0; b < sched.n_blocks(); ++b) {
     const int32_t r0 = sched.blk_ptr[b], r1 = sched.blk_ptr[b + 1];
     // (the stage kernels read row r0 of every block unconditionally)
-    if (r1 <= r0) throw FsError(FS_ERR_BAD_INPUT, "ILU schedule with an empty block");
+    if (r1 <= r0) { bad |= 1; continue; }
     for (int32_t r = r0; r < r1; ++r) {
       const int64_t base = F.rp[r], len = F.rp[r + 1] - base;
       const int32_t* c = F.ci.data() + base;
@@ -199,7 +203,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       const int32_t olint = (int32_t)(std::lower_bound(c, e, r0) - c);
       const int32_t odiag = (int32_t)(std::lower_bound(c, e, r) - c);
       const int32_t ouext = (int32_t)(std::lower_bound(c, e, r1) - c);
-      if (odiag >= len || c[odiag] != r) throw FsError(FS_ERR_BAD_INPUT, "ILU pattern lacks a diagonal");
+      if (odiag >= len || c[odiag] != r) { bad |= 2; break; }
       ol[r] = olint;
       od[r] = odiag;
       ou[r] = ouext;
@@ -207,21 +211,25 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       for (int32_t p = odiag + 1; p < ouext; ++p) uc[c[p] + 1]++;
       sum_lext += olint;
       sum_uext += len - ouext;
-      max_len = std::max<int>(max_len, (int)len);
+      mlen = std::max<int>(mlen, (int)len);
     }
   }
+  if (bad & 1) throw FsError(FS_ERR_BAD_INPUT, "ILU schedule with an empty block");
+  if (bad & 2) throw FsError(FS_ERR_BAD_INPUT, "ILU pattern lacks a diagonal");
+  max_len = mlen;
   // numeric work of the IKJ factorisation: sum over L entries (i, k) of the
   // length of row k's U part (each such entry streams that U part once)
   {
+    std::vector<int32_t> ulen(n);  // U entries per row (a small table the sum reads at random)
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) ulen[i] = (int32_t)(F.rp[i + 1] - F.rp[i] - od[i] - 1);
     int64_t upd = 0;
 #pragma omp parallel for schedule(dynamic, 4096) reduction(+ : upd)
     for (int64_t i = 0; i < n; ++i)
-      for (int64_t p = F.rp[i]; p < F.rp[i] + od[i]; ++p) {
-        const int32_t kk = F.ci[p];
-        upd += F.rp[kk + 1] - F.rp[kk] - od[kk] - 1;
-      }
+      for (int64_t p = F.rp[i]; p < F.rp[i] + od[i]; ++p) upd += ulen[F.ci[p]];
     ikj_updates = upd;
   }
+  fs::phase_mark("ILU: row offsets, IKJ work");
   // per stage: the longest row and L part (the numeric kernel's LDS), and
   // the longest U part of any row up to the stage's end (the rows its L
   // columns can name: the numeric kernel's loads in flight)
@@ -260,6 +268,18 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
       const int64_t l0 = F.rp[r] - F.rp[r - 1], l1 = F.rp[r + 1] - F.rp[r];
       same[r] = blk_of[r] == blk_of[r - 1] && l0 == l1 &&
                 std::equal(F.ci.begin() + F.rp[r], F.ci.begin() + F.rp[r + 1], F.ci.begin() + F.rp[r - 1]);
+    }
+    // node triples as sources: rows k, k+1, k+2 that form one such group
+    // (the stage updates apply their three L columns in one step)
+    {
+      std::vector<uint8_t> tr(n, 0);
+      for (int64_t r = 0; r < n;) {
+        int g = 1;
+        while (g < 3 && r + g < n && same[r + g]) ++g;
+        tr[r] = g == 3;
+        r += g;
+      }
+      tri.upload(tr.data(), tr.size());
     }
     static const int caps[] = {256, 512, 768, 1024, 1536, 2048, 2560, 3072, 4096, 5120, 6144, 8192};
     constexpr int NC = sizeof(caps) / sizeof(caps[0]);
@@ -312,6 +332,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     lc[i + 1] += lc[i];
     uc[i + 1] += uc[i];
   }
+  fs::phase_mark("ILU: stage bins");
   n_lin = lc[n];
   n_uin = uc[n];
   for (int b = 0; b < sched.n_blocks(); ++b) {
@@ -343,6 +364,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   }
   max_rows = std::max(1, (int)sched.max_block_rows);
   if (max_rows > 65535) throw FsError(FS_ERR_UNSUPPORTED, "ILU block larger than 65535 rows");
+  fs::phase_mark("ILU: in-block column lists");
   // LDS budget: one block per workgroup, <= 48 KiB so several workgroups fit a CU
   const int per_wave_budget = 48 * 1024;
   const int fixed = (int)lds_bytes_per_wave(max_rows, 0, tsz_max);
@@ -377,6 +399,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   uv64.alloc(8 * upo.size());
   lv32.alloc(4 * lpo.size());
   uv32.alloc(4 * upo.size());
+  fs::phase_mark("ILU: CSR and in-block uploads");
   // packed in-block inverses and block-contiguous external entries for the
   // dense stage kernel (blocks of at most 256 rows: one block's right-hand
   // side sits in LDS)
@@ -437,6 +460,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         c_max = std::max<int>(c_max, (int)(q - qb));
       }
       n_ent = 4 * q;
+      fs::phase_mark("  ext: chunk layout");
       // the block's halo: its distinct external columns, ascending (nearby
       // lanes gather nearby values); every entry stores its column's index in
       // the halo (1 byte when no halo exceeds 256 columns, else 2); halos
@@ -490,6 +514,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         }
       }
       if (too_big) throw FsError(FS_ERR_UNSUPPORTED, "ILU block with too many external columns");
+      fs::phase_mark("  ext: halos and indices (parallel)");
       h_max = hm;
       ph_max = std::max(ph_max, phm);
       std::vector<int64_t> hoff(nb + 1, 0);
@@ -513,6 +538,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         std::vector<int32_t>().swap(bh[b]);
       }
       n_halo = (int64_t)hc.size();
+      fs::phase_mark("  ext: halo list");
       ib = std::max(h_max, ph_max) <= 256 ? 1 : 2;
       if (ib == 1) {
         std::vector<uint8_t> ix(loc.size());
@@ -535,8 +561,10 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     xl_phmax = xu_phmax = 0;
     ext(true, xldesc, xlrng, xlidx, xlhcol, xlhalo, xlv64, xlv32, n_xl, xl_max, dl, xl_ib, xl_hmax, xl_phmax, n_xlh,
         pl);
+  fs::phase_mark("ILU: external L entries and halos");
     ext(false, xudesc, xurng, xuidx, xuhcol, xuhalo, xuv64, xuv32, n_xu, xu_max, du, xu_ib, xu_hmax, xu_phmax, n_xuh,
         pu);
+  fs::phase_mark("ILU: external U entries and halos");
     // blocks with many external chunks are summed in pieces of `piece`
     // chunks by workgroups of their own; pieces listed stage by stage
     int64_t part_need = 0;
@@ -576,6 +604,7 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
     };
     split(dl, pl, xlsplit, xlpieces, xl_pc_ptr);
     split(du, pu, xusplit, xupieces, xu_pc_ptr);
+  fs::phase_mark("ILU: pieces");
     dpart.alloc(8 * std::max<int64_t>(1, part_need));
     // the last stage has no external U entries when no two blocks of a stage
     // couple: its backward solve is then in-block only and runs in the same
@@ -603,7 +632,8 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
   if (!rp.p || !ci.p || !v64.p || !o_lint.p || !o_diag.p || !o_uext.p || !blk_ptr.p || !stage_ptr.p || !lcp.p ||
       !ucp.p || !lrow.p || !urow.p || !lpos.p || !upos.p || !lv64.p || !uv64.p ||
       (dense && (!inv_off.p || !linv64.p || !uinv64.p || !xldesc.p || !xudesc.p || !xlrng.p || !xurng.p ||
-                 !xlv64.p || !xuv64.p || !xlv32.p || !xuv32.p || !dpart.p)))
+                 !xlv64.p || !xuv64.p || !xlv32.p || !xuv32.p || !dpart.p)) ||
+      (staged && !tri.p))
     throw FsError(FS_ERR_HIP, "ILU factor: device arrays missing after the symbolic phase");
   if (keep_pattern) host_pattern = std::move(F);
 }
@@ -688,7 +718,8 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     if (lds > 64 * 1024)                                                                                       \
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
                 "hipFuncSetAttribute");                                                                        \
-    hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, cnt_rows, ca, cb, nb.mcap, nb.hs);        \
+    hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, tri.as<uint8_t>(), cnt_rows, ca, cb,       \
+                       nb.mcap, nb.hs);                                                                        \
   }
         if (nb.mcap <= 1024) {
           if (nb.G == 3) FS_UPD(256, 2, 3) else if (nb.G == 2) FS_UPD(256, 2, 2) else FS_UPD(256, 2, 1)
@@ -991,7 +1022,8 @@ void DevCsr::upload_pattern(const HostCsr& h, bool with_padded) {
   for (int64_t r = 0; r < n_rows; ++r) r4[r + 1] = r4[r] + (h.rp[r + 1] - h.rp[r] + 3) / 4;
   nnz4 = r4[n_rows];
   std::vector<int32_t> c4(4 * std::max<int64_t>(nnz4, 1), 0);
-  pad_pos.assign(nnz, 0);
+  std::vector<int64_t> pad_pos(std::max<int64_t>(nnz, 1), 0);  // entry -> padded position
+#pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < n_rows; ++r) {
     const int64_t b = h.rp[r], e = h.rp[r + 1];
     int64_t o = 4 * r4[r];
@@ -1004,9 +1036,11 @@ void DevCsr::upload_pattern(const HostCsr& h, bool with_padded) {
   }
   rp4.upload(r4.data(), 8 * (n_rows + 1));
   ci4.upload(c4.data(), 4 * c4.size());
-  v4_64.alloc(8 * c4.size());
-  v4_32.alloc(4 * c4.size());
-  pad_host.assign(c4.size(), 0.0);
+  d_pad_pos.upload(pad_pos.data(), 8 * pad_pos.size());
+  n_pad = (int64_t)c4.size();
+  v4_64.alloc(8 * n_pad);
+  v4_32.alloc(4 * n_pad);
+  hip_check(hipMemset(v4_64.p, 0, 8 * n_pad), "memset");  // padding entries stay 0
   const double g = n_rows ? (double)nnz4 / n_rows : 1;  // groups per row, about two per lane
   G4 = g > 64 ? 32 : g > 32 ? 16 : g > 12 ? 8 : 4;
 }
@@ -1039,36 +1073,50 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
   }
   n_ent = 4 * q;
   // the blocks' column lists and every entry's index into its block's list
-  std::vector<int32_t> cl, hb, loc(std::max<int64_t>(4, n_ent), 0), stamp(std::max<int64_t>(1, h.n_cols), -1),
-    slot(std::max<int64_t>(1, h.n_cols), 0);
-  pos.assign(nnz, 0);
+  std::vector<int32_t> cl, loc(std::max<int64_t>(4, n_ent), 0);
+  std::vector<int64_t> pos(std::max<int64_t>(1, nnz), 0);  // entry -> padded position
+  std::vector<std::vector<int32_t>> bl(nb);               // per block: its column list
+  int bad = 0;
+  // blocks are independent: column lists and indices in parallel
+#pragma omp parallel reduction(| : bad)
+  {
+    std::vector<int32_t> stamp(std::max<int64_t>(1, h.n_cols), -1), slot(std::max<int64_t>(1, h.n_cols), 0);
+#pragma omp for schedule(dynamic, 16)
+    for (int b = 0; b < nb; ++b) {
+      const int32_t r0 = blk_ptr[b], r1 = blk_ptr[b + 1];
+      std::vector<int32_t>& hb = bl[b];
+      for (int64_t e = h.rp[r0]; e < h.rp[r1]; ++e)
+        if (stamp[h.ci[e]] != b) {
+          stamp[h.ci[e]] = b;
+          hb.push_back(h.ci[e]);
+        }
+      std::sort(hb.begin(), hb.end());
+      if (hb.empty()) bad |= 1;
+      if (hb.size() > 65536) bad |= 2;
+      if (bad) continue;
+      for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
+      int64_t o = 4 * (int64_t)dsc[4 * b + 2];
+      for (int32_t r = r0; r < r1; ++r) {
+        const int64_t o0 = o;
+        for (int64_t e = h.rp[r]; e < h.rp[r + 1]; ++e, ++o) {
+          loc[o] = slot[h.ci[e]];
+          pos[e] = o;
+        }
+        for (; (o - o0) % 4; ++o) loc[o] = 0;
+      }
+    }
+  }
+  if (bad & 1) throw FsError(FS_ERR_BAD_INPUT, "SpMV blocking: a block without entries");
+  if (bad & 2) throw FsError(FS_ERR_UNSUPPORTED, "matrix block with too many columns");
   cmax = 0;
   for (int b = 0; b < nb; ++b) {
-    const int32_t r0 = blk_ptr[b], r1 = blk_ptr[b + 1];
-    hb.clear();
-    for (int64_t e = h.rp[r0]; e < h.rp[r1]; ++e)
-      if (stamp[h.ci[e]] != b) {
-        stamp[h.ci[e]] = b;
-        hb.push_back(h.ci[e]);
-      }
-    std::sort(hb.begin(), hb.end());
-    if (hb.empty()) throw FsError(FS_ERR_BAD_INPUT, "SpMV blocking: a block without entries");
-    if (hb.size() > 65536) throw FsError(FS_ERR_UNSUPPORTED, "matrix block with too many columns");
-    for (size_t i = 0; i < hb.size(); ++i) slot[hb[i]] = (int32_t)i;
-    int64_t o = 4 * (int64_t)dsc[4 * b + 2];
-    for (int32_t r = r0; r < r1; ++r) {
-      const int64_t o0 = o;
-      for (int64_t e = h.rp[r]; e < h.rp[r + 1]; ++e, ++o) {
-        loc[o] = slot[h.ci[e]];
-        pos[e] = o;
-      }
-      for (; (o - o0) % 4; ++o) loc[o] = 0;
-    }
+    const std::vector<int32_t>& hb = bl[b];
     ci[2 * b] = (int32_t)cl.size();
     ci[2 * b + 1] = (int32_t)hb.size();
     cmax = std::max<int>(cmax, (int)hb.size());
     cl.insert(cl.end(), hb.begin(), hb.end());
-    while (cl.size() % 4) cl.push_back(hb.empty() ? 0 : hb[0]);
+    while (cl.size() % 4) cl.push_back(hb[0]);
+    std::vector<int32_t>().swap(bl[b]);
   }
   if (cl.empty()) cl.assign(4, 0);
   if (cl.size() > (size_t)INT32_MAX) throw FsError(FS_ERR_UNSUPPORTED, "matrix with too many block columns");
@@ -1082,10 +1130,12 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
   ib = cmax <= 256 ? 1 : 2;
   if (ib == 1) {
     std::vector<uint8_t> ix(loc.size());
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < loc.size(); ++e) ix[e] = (uint8_t)loc[e];
     idx.upload(ix.data(), ix.size());
   } else {
     std::vector<uint16_t> ix(loc.size());
+#pragma omp parallel for schedule(static)
     for (size_t e = 0; e < loc.size(); ++e) ix[e] = (uint16_t)loc[e];
     idx.upload(ix.data(), 2 * ix.size());
   }
@@ -1093,18 +1143,11 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
   cinfo.upload(ci.data(), 4 * ci.size());
   rng.upload(rg.data(), 4 * rg.size());
   cols.upload(cl.data(), 4 * cl.size());
-  pad.assign(std::max<int64_t>(4, n_ent), 0.0);
-  v64.alloc(8 * pad.size());
-  v32.alloc(4 * pad.size());
-}
-
-void BlkCsr::upload_values(const double* vals, hipStream_t st) {
-  if (nb == 0) return;
-  for (int64_t e = 0; e < nnz; ++e) pad[pos[e]] = vals[e];
-  v64.upload(pad.data(), 8 * pad.size());
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad.size())), dim3(BLOCK), 0, st,
-                     (int64_t)pad.size(), v64.as<double>(), v32.as<float>());
-  hip_check(hipStreamSynchronize(st), "sync");
+  d_pos.upload(pos.data(), 8 * pos.size());
+  n_pad = std::max<int64_t>(4, n_ent);
+  v64.alloc(8 * n_pad);
+  v32.alloc(4 * n_pad);
+  hip_check(hipMemset(v64.p, 0, 8 * n_pad), "memset");  // padding entries stay 0
 }
 
 // dst[map[e]] = src[e]
@@ -1117,14 +1160,10 @@ static __global__ __launch_bounds__(BLOCK) void k_scatter_map(int64_t n, const i
 
 void BlkCsr::upload_values_device(const double* d_vals, hipStream_t st) {
   if (nb == 0) return;
-  if (!d_pos.p) {
-    d_pos.upload(pos.data(), 8 * std::max<int64_t>(1, nnz));
-    hip_check(hipMemsetAsync(v64.p, 0, 8 * pad.size(), st), "memset");  // padding entries stay 0
-  }
   hipLaunchKernelGGL(k_scatter_map, dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, d_pos.as<int64_t>(), d_vals,
                      v64.as<double>());
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad.size())), dim3(BLOCK), 0, st,
-                     (int64_t)pad.size(), v64.as<double>(), v32.as<float>());
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n_pad)), dim3(BLOCK), 0, st, n_pad,
+                     v64.as<double>(), v32.as<float>());
 }
 
 void DevCsr::derive_from_v64(hipStream_t st) {
@@ -1132,27 +1171,16 @@ void DevCsr::derive_from_v64(hipStream_t st) {
   hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, v64.as<double>(),
                      v32.as<float>());
   if (!padded) return;
-  if (!d_pad_pos.p) {
-    d_pad_pos.upload(pad_pos.data(), 8 * std::max<int64_t>(1, nnz));
-    hip_check(hipMemsetAsync(v4_64.p, 0, 8 * pad_host.size(), st), "memset");  // padding entries stay 0
-  }
   hipLaunchKernelGGL(k_scatter_map, dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, d_pad_pos.as<int64_t>(),
                      (const double*)v64.as<double>(), v4_64.as<double>());
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad_host.size())), dim3(BLOCK), 0, st,
-                     (int64_t)pad_host.size(), v4_64.as<double>(), v4_32.as<float>());
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n_pad)), dim3(BLOCK), 0, st, n_pad,
+                     v4_64.as<double>(), v4_32.as<float>());
 }
 
 void DevCsr::upload_values(const double* vals, hipStream_t st) {
-  v64.upload(vals, 8 * std::max<int64_t>(1, nnz));
-  if (!v32.p) v32.alloc(4 * std::max<int64_t>(1, nnz));
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, v64.as<double>(),
-                     v32.as<float>());
-  if (padded) {
-    for (int64_t e = 0; e < nnz; ++e) pad_host[pad_pos[e]] = vals[e];
-    v4_64.upload(pad_host.data(), 8 * pad_host.size());
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for((int64_t)pad_host.size())), dim3(BLOCK), 0, st,
-                       (int64_t)pad_host.size(), v4_64.as<double>(), v4_32.as<float>());
-  }
+  if (!v64.p) v64.alloc(8 * std::max<int64_t>(1, nnz));
+  hip_check(hipMemcpyAsync(v64.p, vals, 8 * nnz, hipMemcpyHostToDevice, st), "H2D");
+  derive_from_v64(st);
   hip_check(hipStreamSynchronize(st), "sync");
 }
 

@@ -68,13 +68,12 @@ struct DevCsr {
   int64_t nnz4 = 0;  // groups of 4
   int G4 = 4;
   DevBuf rp4, ci4, v4_64, v4_32;
-  std::vector<int64_t> pad_pos;  // entry -> padded position
-  std::vector<double> pad_host;
+  int64_t n_pad = 0;  // entries of the padded layout
   void upload_pattern(const HostCsr& h, bool with_padded = false);
   // values in the plain layout (host), both precisions and both layouts
   void upload_values(const double* vals, hipStream_t st);
   // v64 already holds the values (written on the device): the FP32 copy and
-  // the padded layout from it (entry -> padded position kept on the device)
+  // the padded layout from it (entry -> padded position, on the device)
   void derive_from_v64(hipStream_t st);
   DevBuf d_pad_pos;
 };
@@ -87,13 +86,11 @@ struct BlkCsr {
   int64_t n = 0, nnz = 0, n_ent = 0, n_cols = 0;  // rows, entries, padded entries, column-list entries
   int nb = 0, ib = 1, cmax = 0, cnmax = 0, rmax = 0;
   DevBuf desc, cinfo, rng, idx, cols, v64, v32;
-  std::vector<int64_t> pos;  // entry -> padded position
-  std::vector<double> pad;   // host staging of the padded values
+  int64_t n_pad = 0;  // entries of the padded layout
   void build(const HostCsr& h, const std::vector<int32_t>& blk_ptr);
-  void upload_values(const double* vals, hipStream_t st);
-  // values in the plain layout already in device memory
+  // values in the plain layout, in device memory
   void upload_values_device(const double* d_vals, hipStream_t st);
-  DevBuf d_pos;
+  DevBuf d_pos;  // entry -> padded position
 };
 
 struct IluDev {
@@ -121,6 +118,7 @@ struct IluDev {
   bool need_v32 = true;      // FP32 CSR copy of the factor (substitution kernels)
   bool keep_pattern = true;  // keep the host pattern (factor queries, transposed factor)
   std::vector<NumBin> bins;
+  DevBuf tri;  // [n] row starts a node triple (staged factorisation only)
   // explicit inverses of the in-block triangles (packed per block), used by
   // apply(); `dense` off keeps the in-block substitution kernels
   bool dense = true;

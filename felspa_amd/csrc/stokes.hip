@@ -236,10 +236,11 @@ struct fs_handle {
   hipStream_t st = nullptr;
   DevCsr K, Mp, Ablk;                                     // K, M_p and A alone, solver order
   BlkCsr Ablb;                                            // A by the ILU's lattice blocks (inner PCG SpMV)
-  std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp;  // caller entry -> K / Mp position
-  std::vector<int64_t> ablk_from_k;                       // A entry -> K position
-  std::vector<double> hK, hMp, hA;                        // host value staging
-  DevBuf dmapA, dmapBt, dmapB, dmapK11, dmapMp, dablk;   // the maps above on the device (device refactor)
+  // value maps, caller entry -> K / Mp position and A entry -> K position:
+  // built on the host at create, then kept on the device only
+  std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp, ablk_from_k;
+  int64_t nmapA = 0, nmapBt = 0, nmapB = 0, nmapK11 = 0, nmapMp = 0, nablk = 0;
+  DevBuf dmapA, dmapBt, dmapB, dmapK11, dmapMp, dablk;
   IluDev FA, FS;
   DevBuf b, x, r, V, Z, ytmp, tp, cg, partial, sc, gbar, flags, perm_all, vin, pcg, pcg_part;
   int64_t ldu = 0;
@@ -628,48 +629,61 @@ struct fs_handle {
     dirty = false;
   }
 
-  void upload_values() {
-    K.upload_values(hK.data(), st);
-    for (size_t e = 0; e < ablk_from_k.size(); ++e) hA[e] = hK[ablk_from_k[e]];
-    Ablk.upload_values(hA.data(), st);
-    Ablb.upload_values(hA.data(), st);
-    Mp.upload_values(hMp.data(), st);
-    dirty = true;
+  // the value maps to the device; the host copies are released
+  void maps_to_device() {
+    auto up = [](DevBuf& d, std::vector<int64_t>& m, int64_t& cnt) {
+      cnt = (int64_t)m.size();
+      d.upload(m.data(), 8 * std::max<size_t>(1, m.size()));
+      std::vector<int64_t>().swap(m);
+    };
+    up(dmapA, mapA, nmapA);
+    up(dmapBt, mapBt, nmapBt);
+    up(dmapB, mapB, nmapB);
+    up(dmapK11, mapK11, nmapK11);
+    up(dmapMp, mapMp, nmapMp);
+    up(dablk, ablk_from_k, nablk);
   }
 
+  // New values in host memory (the caller's layouts): copied to the device
+  // as they are (one staging buffer, released afterwards), then scattered
+  // there, so no host copy of the solver's layouts is kept.
   void set_values(const double* A, const double* Bt, const double* B, const double* K11, const double* M) {
-    for (size_t e = 0; e < mapA.size(); ++e) hK[mapA[e]] = A[e];
-    for (size_t e = 0; e < mapBt.size(); ++e) hK[mapBt[e]] = Bt[e];
-    for (size_t e = 0; e < mapB.size(); ++e) hK[mapB[e]] = B[e];
-    for (size_t e = 0; e < mapK11.size(); ++e) hK[mapK11[e]] = K11[e];
-    for (size_t e = 0; e < mapMp.size(); ++e) hMp[mapMp[e]] = M[e];
-    upload_values();
+    const int64_t tot = nmapA + nmapBt + nmapB + nmapK11 + nmapMp;
+    DevBuf stage;
+    stage.alloc(8 * std::max<int64_t>(1, tot));
+    double* d = stage.as<double>();
+    const double* src[5] = {A, Bt, B, K11, M};
+    const int64_t cnt[5] = {nmapA, nmapBt, nmapB, nmapK11, nmapMp};
+    double* dst[5];
+    int64_t off = 0;
+    for (int i = 0; i < 5; ++i) {
+      dst[i] = d + off;
+      if (cnt[i] > 0)
+        hip_check(hipMemcpyAsync(dst[i], src[i], 8 * cnt[i], hipMemcpyHostToDevice, st), "H2D values");
+      off += cnt[i];
+    }
+    set_values_device(dst[0], dst[1], dst[2], dst[3], dst[4]);  // synchronises the stream
   }
 
   // New values already in device memory (the caller's layouts): scattered
   // into K and M_p by device copies of the value maps, A alone gathered from
-  // K; the maps go to the device on first use.
+  // K.
   void set_values_device(const double* A, const double* Bt, const double* B, const double* K11, const double* M) {
-    if (!dmapA.p) {
-      dmapA.upload(mapA.data(), 8 * std::max<size_t>(1, mapA.size()));
-      dmapBt.upload(mapBt.data(), 8 * std::max<size_t>(1, mapBt.size()));
-      dmapB.upload(mapB.data(), 8 * std::max<size_t>(1, mapB.size()));
-      dmapK11.upload(mapK11.data(), 8 * std::max<size_t>(1, mapK11.size()));
-      dmapMp.upload(mapMp.data(), 8 * std::max<size_t>(1, mapMp.size()));
-      dablk.upload(ablk_from_k.data(), 8 * std::max<size_t>(1, ablk_from_k.size()));
-    }
-    auto scatter = [&](const DevBuf& map, size_t n, const double* src, DevBuf& dst) {
+    auto scatter = [&](const DevBuf& map, int64_t n, const double* src, DevBuf& dst) {
       if (n == 0) return;
-      hipLaunchKernelGGL(k_values_scatter, dim3(grid((int64_t)n)), dim3(BLOCK), 0, st, (int64_t)n,
-                         map.as<int64_t>(), src, dst.as<double>());
+      hipLaunchKernelGGL(k_values_scatter, dim3(grid(n)), dim3(BLOCK), 0, st, n, map.as<int64_t>(), src,
+                         dst.as<double>());
     };
-    scatter(dmapA, mapA.size(), A, K.v64);
-    scatter(dmapBt, mapBt.size(), Bt, K.v64);
-    scatter(dmapB, mapB.size(), B, K.v64);
-    scatter(dmapK11, mapK11.size(), K11, K.v64);
-    scatter(dmapMp, mapMp.size(), M, Mp.v64);
+    if (!K.v64.p) K.v64.alloc(8 * std::max<int64_t>(1, K.nnz));
+    if (!Mp.v64.p) Mp.v64.alloc(8 * std::max<int64_t>(1, Mp.nnz));
+    if (!Ablk.v64.p) Ablk.v64.alloc(8 * std::max<int64_t>(1, Ablk.nnz));
+    scatter(dmapA, nmapA, A, K.v64);
+    scatter(dmapBt, nmapBt, Bt, K.v64);
+    scatter(dmapB, nmapB, B, K.v64);
+    scatter(dmapK11, nmapK11, K11, K.v64);
+    scatter(dmapMp, nmapMp, M, Mp.v64);
     K.derive_from_v64(st);
-    const int64_t na = (int64_t)ablk_from_k.size();
+    const int64_t na = nablk;
     if (na > 0)
       hipLaunchKernelGGL(k_values_gather, dim3(grid(na)), dim3(BLOCK), 0, st, na, dablk.as<int64_t>(),
                          (const double*)K.v64.as<double>(), Ablk.v64.as<double>());
@@ -1014,8 +1028,9 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
       }
     }
   }
+  fs::phase_mark("K in solver order: rows and value maps");
   h.K.upload_pattern(Kh, true);
-  h.hK.assign(Kh.nnz(), 0.0);
+  fs::phase_mark("K: device pattern + padded copy");
   h.diag_only.clear();
   for (int64_t r = 0; r < n; ++r)
     if (Kh.rp[r + 1] - Kh.rp[r] == 1 && Kh.ci[Kh.rp[r]] == r) h.diag_only.push_back((int32_t)r);
@@ -1023,21 +1038,26 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
   HostCsr Ah;
   Ah.n_rows = Ah.n_cols = nu;
   Ah.rp.assign(nu + 1, 0);
+#pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < nu; ++r) {
     int64_t c = 0;
     for (int64_t q = Kh.rp[r]; q < Kh.rp[r + 1]; ++q) c += Kh.ci[q] < nu;
-    Ah.rp[r + 1] = Ah.rp[r] + c;
+    Ah.rp[r + 1] = c;
   }
+  for (int64_t r = 0; r < nu; ++r) Ah.rp[r + 1] += Ah.rp[r];
   Ah.ci.resize(Ah.rp[nu]);
   h.ablk_from_k.resize(Ah.rp[nu]);
+#pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < nu; ++r) {
     int64_t o = Ah.rp[r];
     for (int64_t q = Kh.rp[r]; q < Kh.rp[r + 1]; ++q)
       if (Kh.ci[q] < nu) { Ah.ci[o] = Kh.ci[q]; h.ablk_from_k[o] = q; ++o; }
   }
+  fs::phase_mark("A alone in solver order");
   h.Ablk.upload_pattern(Ah, true);
+  fs::phase_mark("A: device pattern + padded copy");
   h.Ablb.build(Ah, h.FA.sched.blk_ptr);
-  h.hA.assign(Ah.nnz(), 0.0);
+  fs::phase_mark("A by lattice blocks");
 }
 
 void fill_stats_err(fs_stats* s, int code) {
@@ -1195,14 +1215,9 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
     throw FsError(FS_ERR_BAD_INPUT, "block shapes do not match");
   if (h.n >= INT32_MAX) throw FsError(FS_ERR_BAD_INPUT, "system too large");
   // phase times of the set-up (printed with o.verbose)
-  auto t_last = std::chrono::steady_clock::now();
-  auto phase = [&](const char* what) {
-    const auto t = std::chrono::steady_clock::now();
-    if (o.verbose)
-      std::fprintf(stderr, "fs_stokes_create: %-34s %8.2f s\n", what,
-                   std::chrono::duration<double>(t - t_last).count());
-    t_last = t;
-  };
+  fs::g_verbose = o.verbose;
+  fs::phase_reset();
+  auto phase = [](const char* what) { fs::phase_mark(what); };
   // orderings
   const int bmax = 256;
   HostPattern pA = pattern_of(hA), pM = pattern_of(hM);
@@ -1226,9 +1241,9 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   phase("ILU(k) of A: symbolic + device layout");
   h.FS.dense = false;  // M_p's factor is applied inside the fused inner CG only
   h.FS.build(pattern_of(Mnew), sS, o.ilu_k_S, 8);
+  phase("ILU of M_p");
   // K in solver order + value maps from the caller's layout
   build_K(h, A, Bt, B, K11);
-  phase("ILU of M_p, K in solver order");
   h.Mp.upload_pattern(Mnew);
   // mapM is indexed by the sorted-layout entries of hM; convert to caller entries
   {
@@ -1236,7 +1251,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
     h.mapMp.resize(c2s.size());
     for (size_t e = 0; e < c2s.size(); ++e) h.mapMp[e] = mapM[c2s[e]];
   }
-  h.hMp.assign(h.Mp.nnz, 0.0);
+  h.maps_to_device();
   // device vectors
   h.ld = (h.n + 63) & ~int64_t(63);
   // (the method can be switched later: room for either basis)
