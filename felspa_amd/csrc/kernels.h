@@ -1604,11 +1604,12 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_stage(IluView<double> F, int
 // Left-looking ILU by stages. Every external L column of a row (a column in
 // another block) lies in an earlier stage, and in IKJ order a row applies its
 // L columns in ascending order: all external ones, stage by stage, then the
-// in-block ones. So once the rows of stage s are final, every later row can
-// apply its L columns of stage s at once, with its partial values kept in the
-// factor between launches; the per-block kernel above (from_factor) then
-// applies the in-block columns and forms the pivots. Each entry receives the
-// same updates in the same order as in the row-by-row IKJ loop.
+// in-block ones. So once every earlier stage is final, the rows of a stage
+// can apply all their external L columns at once, one workgroup per row
+// group, their values kept in the factor; the per-block kernel above
+// (from_factor) then applies the in-block columns and forms the pivots. Each
+// entry receives the same updates in the same order as in the row-by-row
+// IKJ loop.
 //
 // Source values first: the factor zeroed, then every entry of the source
 // row (columns < col_limit) stored at its position (binary search in the
@@ -1670,24 +1671,25 @@ __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) 
 
 // One stage update: the row groups starting at rows[0 .. nrows) (G
 // consecutive rows of one block with the same columns, e.g. the velocity
-// components of one node; ascending, each row of at most mcap entries, all
-// after the stage) apply their external L columns in [col_a, col_b), the
-// rows of the finished stage. One workgroup of NT threads per group,
-// workgroups XCD-contiguous so that neighbouring rows, which name the same
-// finished rows, share an L2. The rows from their first such column on go to
+// components of one node; ascending, each row of at most mcap entries) apply
+// their L columns in [col_a, col_b), rows of finished stages. One workgroup
+// of NT threads per group, workgroups XCD-contiguous so that neighbouring
+// rows, which name the same finished rows, share an L2. The rows from their first such column on go to
 // LDS (values per row, the shared columns and hash map); then the L columns
 // in order, in batches of 64 whose U spans, pivots and first U values are
 // read first. A step is one L column k, or three, k, k+1, k+2, when those
 // rows start a node triple (tri[k]: three consecutive rows of one block with
 // the same columns, so row k's U part starts with k+1, k+2 and row k+1's
 // with k+2, and past k+2 all three U parts name the same columns): the
-// step's multipliers m = w_t / u_kk (for a triple: with the in-node updates
+// step's multipliers m = w_t (1 / u_kk) (for a triple: with the in-node updates
 // of w_{t+1}, w_{t+2} first) go to the factor, then one stream of the U
 // part(s) (chunks of NT M entries, loaded three chunks ahead, each column
 // looked up once for the G rows and the step's 1 or 3 source rows) updates
 // w_j -= m_k u_kj (-= m_k+1 u_k+1,j -= m_k+2 u_k+2,j) where the rows store j.
 // Every w_j receives the same fused multiply-adds in the same order as in
-// the column-by-column IKJ loop. Then the rest goes back to the factor.
+// the column-by-column IKJ loop; a multiplier is w_t times the pivot's
+// reciprocal (taken once per column and batch), within an ulp of w_t / u_kk.
+// Then the rest goes back to the factor.
 template <int NT, int M, int G>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
                                                            const uint8_t* __restrict__ tri, int nrows,
@@ -1695,7 +1697,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
   extern __shared__ __attribute__((aligned(16))) char smem[];
   double* w = (double*)smem;                     // [G][mcap + 1]; w[g][mcap]: dump slot
   int64_t* mq = (int64_t*)(w + G * (mcap + 1));  // [UPD_BATCH] first U entry of row k
-  double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] pivot u_kk
+  double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] 1 / u_kk
   double* mf = mu + UPD_BATCH;                    // [UPD_BATCH] first U value of row k
   double* ms = mf + UPD_BATCH;                    // [UPD_BATCH] second U value of row k
   int32_t* mn = (int32_t*)(ms + UPD_BATCH);       // [UPD_BATCH] U entries of row k
@@ -1748,7 +1750,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
       const int64_t q = kb + kd + 1;
       mq[tid] = q;
       mn[tid] = nu_k;
-      mu[tid] = F.v[kb + kd];
+      mu[tid] = 1.0 / F.v[kb + kd];  // the pivot's reciprocal, taken once per column
       mf[tid] = F.v[nu_k > 0 ? q : kb + kd];
       ms[tid] = F.v[nu_k > 1 ? q + 1 : kb + kd];
       tf[tid] = tri[k] != 0 && tid + 2 < nb && lc[b0 + tid + 1] == k + 1 && lc[b0 + tid + 2] == k + 2;
@@ -1819,12 +1821,12 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
         const int t = b0 + sl;
         if (t3) {
           const double u01 = mf[sl], u02 = ms[sl], u12 = mf[sl + 1];
-          const double p0 = mu[sl], p1 = mu[sl + 1], p2 = mu[sl + 2];
+          const double r0 = mu[sl], r1 = mu[sl + 1], r2 = mu[sl + 2];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            m0[g] = w[g * ld + t] / p0;
-            m1[g] = fma(-m0[g], u01, w[g * ld + t + 1]) / p1;
-            m2[g] = fma(-m1[g], u12, fma(-m0[g], u02, w[g * ld + t + 2])) / p2;
+            m0[g] = w[g * ld + t] * r0;
+            m1[g] = fma(-m0[g], u01, w[g * ld + t + 1]) * r1;
+            m2[g] = fma(-m1[g], u12, fma(-m0[g], u02, w[g * ld + t + 2])) * r2;
           }
 #pragma unroll
           for (int g = 0; g < G; ++g) {
@@ -1833,10 +1835,10 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
             if (tid == 3 * g + 2) F.v[bg[g] + t + 2] = m2[g];
           }
         } else {
-          const double piv = mu[sl];
+          const double rp = mu[sl];
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            m0[g] = w[g * ld + t] / piv;
+            m0[g] = w[g * ld + t] * rp;
             if (tid == g) F.v[bg[g] + t] = m0[g];
           }
         }

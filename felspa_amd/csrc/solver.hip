@@ -677,6 +677,41 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
   for (int s = 0; s < sched.n_stages(); ++s) {
     const int b0 = sched.stage_ptr[s], b1 = sched.stage_ptr[s + 1];
     if (b1 <= b0) continue;
+    if (staged && s > 0) {
+      // every external L column of the stage's rows (all in earlier,
+      // finished stages), applied in ascending order by one workgroup per
+      // row group before the stage's own kernel: each row is loaded, hashed
+      // and stored once. The update is latency-bound and its LDS allows two
+      // (long rows) to four workgroups per CU, so the workgroups are wide:
+      // 768 threads x 1 entry per chunk for rows above 1024 entries, 256 x 2
+      // below (at 48^3, ILU(2), one launch per finished stage for all later
+      // rows: 256 x 8 took 11.0 s, 256 x 4 10.6 s, 128 x 8 15.8 s, 512 x 2
+      // 8.0 s, 1024 x 1 7.2 s per factorisation; at 32^3 with node-triple
+      // steps 1024 x 1 1.22 s, 512 x 2 1.17 s, 768 x 1 1.14 s)
+      const int32_t cb = sched.blk_ptr[b0];
+      for (const NumBin& nb : bins) {
+        const int first = nb.first[s - 1], cnt_rows = nb.first[s] - first;
+        if (cnt_rows <= 0) continue;
+        const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs, nb.G);
+        const int32_t* rl = nb.rows.as<int32_t>() + first;
+#define FS_UPD(NT, M, G)                                                                                       \
+  {                                                                                                            \
+    auto fn = &k_ilu_numeric_update<NT, M, G>;                                                                 \
+    if (lds > 64 * 1024)                                                                                       \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, tri.as<uint8_t>(), cnt_rows, 0, cb,        \
+                       nb.mcap, nb.hs);                                                                        \
+  }
+        if (nb.mcap <= 1024) {
+          if (nb.G == 3) FS_UPD(256, 2, 3) else if (nb.G == 2) FS_UPD(256, 2, 2) else FS_UPD(256, 2, 1)
+        } else {
+          if (nb.G == 3) FS_UPD(768, 1, 3) else if (nb.G == 2) FS_UPD(768, 1, 2) else FS_UPD(768, 1, 1)
+        }
+#undef FS_UPD
+        cnt.launches++;
+      }
+    }
     // LDS sized by the stage's longest row; threads x loads per thread cover
     // the longest U part its L columns can name (one wave up to 256 entries,
     // four up to 1024, sixteen waves of 2 loads each beyond)
@@ -699,37 +734,6 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     else FS_NUM(1024, 2)
 #undef FS_NUM
     cnt.launches++;
-    if (staged && s + 1 < sched.n_stages()) {
-      // the finished stage's columns into every later row. The update is
-      // latency-bound and its LDS allows two (long rows) to four workgroups
-      // per CU, so the workgroups are wide: 1024 threads x 1 entry per
-      // chunk for rows above 1024 entries, 256 x 2 below (at 48^3, ILU(2):
-      // 256 x 8 took 11.0 s, 256 x 4 10.6 s, 128 x 8 15.8 s, 512 x 2 8.0 s,
-      // 1024 x 1 7.2 s per factorisation)
-      const int32_t ca = sched.blk_ptr[b0], cb = sched.blk_ptr[b1];
-      for (const NumBin& nb : bins) {
-        const int cnt_rows = nb.nrows - nb.first[s];
-        if (cnt_rows <= 0) continue;
-        const size_t lds = ilu_update_lds_bytes(nb.mcap, nb.hs, nb.G);
-        const int32_t* rl = nb.rows.as<int32_t>() + nb.first[s];
-#define FS_UPD(NT, M, G)                                                                                       \
-  {                                                                                                            \
-    auto fn = &k_ilu_numeric_update<NT, M, G>;                                                                 \
-    if (lds > 64 * 1024)                                                                                       \
-      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
-                "hipFuncSetAttribute");                                                                        \
-    hipLaunchKernelGGL(fn, dim3(cnt_rows), dim3(NT), lds, st, V, rl, tri.as<uint8_t>(), cnt_rows, ca, cb,       \
-                       nb.mcap, nb.hs);                                                                        \
-  }
-        if (nb.mcap <= 1024) {
-          if (nb.G == 3) FS_UPD(256, 2, 3) else if (nb.G == 2) FS_UPD(256, 2, 2) else FS_UPD(256, 2, 1)
-        } else {
-          if (nb.G == 3) FS_UPD(1024, 1, 3) else if (nb.G == 2) FS_UPD(1024, 1, 2) else FS_UPD(1024, 1, 1)
-        }
-#undef FS_UPD
-        cnt.launches++;
-      }
-    }
   }
   cnt.bytes += (double)nnz * 12.0 * 2.0;
   hip_check(hipGetLastError(), "ilu numeric launch");

@@ -1695,6 +1695,10 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
         if (f32) h->inner_cg<float>((const float*)h->ytmp.p + nu, (float*)base);
         else h->inner_cg<double>(h->vin.as<double>() + nu, (double*)base);
         break;
+      case 4:  // the numeric ILU(k) of A from K's values (FP64; each repetition the same factor)
+        h->FA.factorize(h->K.rp.as<int64_t>(), h->K.ci.as<int32_t>(), h->K.v64.as<double>(), nu, h->st,
+                        dptr<int>(h->flags, 0), h->cnt);
+        break;
       default:
         throw FsError(FS_ERR_BAD_INPUT, "unknown kernel class");
     }

@@ -240,7 +240,8 @@ int fs_stokes_get_factor_info(fs_handle* h, int32_t which, int64_t* n, int64_t* 
  * inner PCG (q = A p with the p.q and r.p partials), 1 = K SpMV (FP64 true
  * residual), 2 = one ILU(k) application on A as the inner PCG runs it (all
  * stage launches, the forward stages carrying the PCG update step), 3 = one
- * inner CG on M_p. */
+ * inner CG on M_p, 4 = one numeric ILU(k) factorisation of A (FP64, all its
+ * launches and the block inverses; precision ignored). */
 int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t reps, double* ms_per_launch,
                           double* bytes_per_launch);
 
