@@ -81,6 +81,13 @@ def test_default_options_follow_reference():
     assert o.ilu_k_A == 1 and o.ilu_k_S == 0  # set_level_of_fill defaults
 
 
+def test_setters_reject_a_null_handle():
+    lib = L.lib()
+    assert lib.fs_stokes_set_krylov(None, 60, 30, -1.0, -1, -1.0, -1) == L.FS_ERR_BAD_INPUT
+    assert lib.fs_stokes_set_method(None, L.FS_METHOD_FGMRES) == L.FS_ERR_BAD_INPUT
+    assert lib.fs_stokes_set_precision(None, L.FS_PREC_MIXED) == L.FS_ERR_BAD_INPUT
+
+
 def test_create_without_gpu_reports_no_device():
     import torch
     if torch.cuda.is_available():

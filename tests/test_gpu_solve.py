@@ -113,6 +113,33 @@ def test_fgmres_matches_oracle_port(oracle, cfg, n, kA):
     assert du < 1e-6 and dpm < 1e-4, (du, dpm)
 
 
+def test_large_basis_and_krylov_switch_match_oracle_port(oracle):
+    """A Krylov basis past 64 vectors (the Gram-Schmidt multi-dots run in
+    chunks of 32): FP64 FGMRES(200) with one inner PCG step on C1 16^2 needs
+    about 110 outer iterations without a restart; same count (within two) and
+    solution as the oracle's restatement. Then fs_stokes_set_krylov on the
+    same handle: a restart beyond the allocated basis is refused, and
+    FGMRES(30) with 30 inner steps matches the oracle's run of those
+    settings."""
+    S = system("C1", 16)
+    lin = fa.StokesLinearSystem(S.blocks, A_lof=0, S_lof=0, ordering=L.FS_ORDER_REFERENCE, restart=200,
+                                inner_a_iters=1, constraints=S.constrained)
+    for restart, ia, min_iters in ((200, 1, 65), (30, 30, 1)):
+        lin.set_krylov(restart=restart, inner_a_iters=ia)
+        x = np.zeros(S.n)
+        st = lin.solve(x, S.rhs)
+        xo, sto = oracle.stokes_solve(S.blocks, S.rhs, kA=0, kS=0, method=2, restart=restart, rtol=1e-8,
+                                      inner_a_iters=ia)
+        assert sto["status"] == 0 and st["status"] == 0
+        assert st["outer_iters"] >= min_iters, st["outer_iters"]
+        assert abs(st["outer_iters"] - sto["outer_iters"]) <= 2, (restart, st["outer_iters"], sto["outer_iters"])
+        assert rel(x[:S.n_u], xo[:S.n_u]) < 1e-6
+    with pytest.raises(L.FsException) as e:
+        lin.set_krylov(restart=201)
+    assert e.value.code == L.FS_ERR_BAD_INPUT
+    lin.close()
+
+
 @pytest.mark.parametrize("precision", [L.FS_PREC_FP64, L.FS_PREC_MIXED, L.FS_PREC_MIXED_FP64_VECTORS])
 @pytest.mark.parametrize("cfg,n,kA", CASES)
 def test_fgmres_tight_tolerance(cfg, n, kA, precision):
