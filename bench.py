@@ -54,8 +54,9 @@ def cpu_baseline_leg(lin, S, kA, plain_stats, inner_a, threads):
       ILU application (serial, as the reference's vmult): on that block,
         scaled by nnz(full factor) / nnz(block factor);
       SpMV with K and with A: the full matrices, `threads` threads.
-    Estimate = numeric + outer (inner_a (ILU apply + A SpMV) + K SpMV). The
-    Arnoldi vector work and the inner CG on M_p are left out (a lower bound)."""
+    Estimate = numeric + outer (max(1, inner_a) ILU apply + inner_a A SpMV +
+    K SpMV). The Arnoldi vector work and the inner CG on M_p are left out (a
+    lower bound)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     oracle_py.build()
@@ -107,9 +108,10 @@ def cpu_baseline_leg(lin, S, kA, plain_stats, inner_a, threads):
     num = out["t_num"] * fi["ikj_updates"] / max(1, out["upd"])
     app = out["t_app"] * fi["nnz"] / max(1, out["nnz"])
     outer = plain_stats["outer_iters"]
-    est = num + outer * (inner_a * (app + t_a) + t_k)
+    est = num + outer * (max(1, inner_a) * app + inner_a * t_a + t_k)
     return {"value": round(est * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": ("oracle FP64 restatement of the same method (FGMRES, %d inner ILU(%d)-PCG steps on A), "
+            "sample": ("oracle FP64 restatement of the same method (FGMRES, %d inner ILU(%d)-PCG steps on A; "
+                       "0: one ILU application), "
                        "timed on bounded samples of this system and scaled: numeric ILU(%d) on the leading "
                        "%d rows of P A P^T (%d of %d colour stages) %.2f s for %.3e IKJ updates -> %.1f s for "
                        "the full factor's %.3e; one serial ILU application %.4f s on %d factor entries -> "
@@ -339,8 +341,11 @@ def main():
                                  reps=args.kernel_reps)
         timing[kind] = (ms, by)
         log("kernel %-64s %.4f ms/unit  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
-    pcg_steps = st_head["outer_iters"] * inner_a
-    share = {k: pcg_steps * timing[k][0] / st_head["t_total_ms"] for k in (0, 2)}
+    # per outer iteration: max(1, inner_a) ILU applications (inner_a = 0: one
+    # application as the reference's A preconditioner) and inner_a SpMVs
+    n_apply = st_head["outer_iters"] * max(1, inner_a)
+    n_spmv = st_head["outer_iters"] * inner_a
+    share = {0: n_spmv * timing[0][0] / st_head["t_total_ms"], 2: n_apply * timing[2][0] / st_head["t_total_ms"]}
     dom = 2 if share[2] >= share[0] else 0
     ms_dom, by_fmt = timing[dom]
     nu = S.n_u

@@ -3,8 +3,9 @@
 Solver settings per workload: ILU levels, lattice nodes per ILU block edge,
 and the flexible method's Krylov basis (restart) and inner ILU-PCG steps on A
 per outer iteration. The last two are the measured optimum on one MI355X
-(mixed precision, FGMRES restart {60, 250} x inner steps 1..60): 2D needs
-15-30 inner steps (C2 stagnates with 8), 3D is fastest with 1-3, where the
+(mixed precision, FGMRES restart {60, 250} x inner steps 0..60; 0 = one ILU
+application per outer iteration, the reference's A preconditioner): 2D needs
+15-30 inner steps (C2 stagnates with 8), 3D is fastest with none, where the
 outer iterations grow less than the inner work shrinks.
 
 C1  2D inclusion, 64x64 Q2-Q1, shear top/bottom + periodic x, ILU(0) (CPU-runnable)
@@ -26,13 +27,13 @@ CONFIGS = {
                gravity=1.0, ilu_k_A=1, ilu_k_S=0, hint_nodes=5, restart=250, inner_a=30),
     "C3": dict(dim=3, n_cells=32, field=L.FS_FIELD_INCLUSION, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, eta_incl=1e3, radius=0.2, ilu_k_A=1, ilu_k_S=0, hint_nodes=3, restart=250,
-               inner_a=3),
+               inner_a=0),
     "C4": dict(dim=2, n_cells=128, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 8), axes=(0.05, 0.2), logc=(0.0, 3.0),
                ilu_k_A=1, ilu_k_S=0, hint_nodes=5, restart=250, inner_a=20),
     "C5": dict(dim=3, n_cells=48, field=L.FS_FIELD_ELLIPSES, bc=L.FS_BC_LINEAR_SHEAR,
                eta_matrix=1.0, n_ellipses=(1, 4), axes=(0.08, 0.25), logc=(1.0, 3.0),
-               ilu_k_A=2, ilu_k_S=0, hint_nodes=3, restart=250, inner_a=3),
+               ilu_k_A=2, ilu_k_S=0, hint_nodes=3, restart=250, inner_a=0),
 }
 
 _SOLVER_KEYS = ("ilu_k_A", "ilu_k_S", "hint_nodes", "restart", "inner_a")

@@ -1659,6 +1659,32 @@ __device__ __forceinline__ int wave_lower_bound(const int32_t* __restrict__ a, i
   return lo + __popcll(__ballot(lt));
 }
 
+// Node-keyed hash map of the stage updates. A 3D velocity row names the
+// columns of a lattice node (its three components 3q, 3q+1, 3q+2) together,
+// so the map is keyed by the node q = column / 3, with a third of the keys of
+// a column-keyed map and shorter probe chains at the same size (a step waits
+// for the longest chain among its threads). Entry: a 15-bit tag above the
+// node's present components (3 bits) above the position of its first
+// present column (13 bits: rows of at most 8192 entries); the present
+// columns of a node are consecutive in the sorted row. Empty slots are all
+// ones (a valid entry has its top bit clear).
+__device__ __forceinline__ uint32_t node_of(int32_t c) { return (uint32_t)c / 3u; }
+__device__ __forceinline__ int node_probe(const uint32_t* tab, const int32_t* lc, uint32_t mask, uint32_t q,
+                                          uint32_t comp, uint32_t tag, uint32_t h, uint32_t e) {
+  while (e != NUM_EMPTY) {
+    if ((e >> 16) == tag) {
+      const int base = (int)(e & 0x1FFFu);
+      if (node_of(lc[base]) == q) {
+        const uint32_t pm = (e >> 13) & 7u;
+        return ((pm >> comp) & 1u) ? base + __popc(pm & ((1u << comp) - 1u)) : -1;
+      }
+    }
+    h = (h + 1) & mask;
+    e = tab[h];
+  }
+  return -1;
+}
+
 // LDS of one stage-update workgroup: the G rows' working values from the
 // first updated column on (and a dump slot each), a batch of 64 L columns'
 // U spans, pivots, first two U values, node-triple flags and steps, the
@@ -1726,14 +1752,22 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
 #pragma unroll
   for (int g = 0; g < G; ++g) bg[g] = F.rp[i + g] + pa;
   for (int h = tid; h < hs; h += NT) tab[h] = NUM_EMPTY;
-  lds_barrier();
   for (int u = tid; u < m; u += NT) {
-    const int32_t c = F.ci[base + pa + u];
-    lc[u] = c;
+    lc[u] = F.ci[base + pa + u];
 #pragma unroll
     for (int g = 0; g < G; ++g) w[g * ld + u] = F.v[bg[g] + u];
-    const uint32_t hh = num_hash(c);
-    const uint32_t ent = ((hh & 0xFFFFu) << 16) | (uint32_t)u;
+  }
+  lds_barrier();
+  // one entry per node, from the node's first present column
+  for (int u = tid; u < m; u += NT) {
+    const int32_t c = lc[u];
+    const uint32_t q = node_of(c);
+    if (u > 0 && node_of(lc[u - 1]) == q) continue;
+    uint32_t pm = 1u << ((uint32_t)c - 3u * q);
+    if (u + 1 < m && node_of(lc[u + 1]) == q) pm |= 1u << ((uint32_t)lc[u + 1] - 3u * q);
+    if (u + 2 < m && node_of(lc[u + 2]) == q) pm |= 1u << ((uint32_t)lc[u + 2] - 3u * q);
+    const uint32_t hh = num_hash((int32_t)q);
+    const uint32_t ent = ((hh & 0x7FFFu) << 16) | (pm << 13) | (uint32_t)u;
     for (uint32_t h = hh >> (32 - lg);; h = (h + 1) & mask)
       if (atomicCAS(&tab[h], NUM_EMPTY, ent) == NUM_EMPTY) break;
   }
@@ -1844,16 +1878,18 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
         }
       }
       const int n = t3 ? mn[sl + 2] : mn[sl];
-      uint32_t h[M], e[M];
+      uint32_t q[M], h[M], e[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        h[r] = num_hash(j[r]) >> (32 - lg);
+        q[r] = node_of(j[r]);
+        h[r] = num_hash((int32_t)q[r]) >> (32 - lg);
         e[r] = tab[h[r]];
       }
       int pos[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
-        const int pp = num_probe(tab, lc, mask, j[r], num_hash(j[r]) & 0xFFFFu, h[r], e[r]);
+        const int pp = node_probe(tab, lc, mask, q[r], (uint32_t)j[r] - 3u * q[r],
+                                  num_hash((int32_t)q[r]) & 0x7FFFu, h[r], e[r]);
         pos[r] = p.y + tid + r * NT < n && pp >= 0 ? pp : mcap;
       }
       if (t3) {
