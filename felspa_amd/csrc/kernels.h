@@ -282,11 +282,13 @@ __global__ __launch_bounds__(BLOCK) void k_spmv4_dot(int64_t nrows, const int64_
 // the G partial sums are added in order.
 template <int IB>
 struct Idx4;
-template <typename TV, typename T, int IB, int NT>
+// TX: the type of p when it differs from T (the product K z of the outer
+// method: FP32 z, FP64 sums and q); partial may then be null (no dot).
+template <typename TV, typename T, int IB, int NT, typename TX = T>
 __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ desc, const int2* __restrict__ cinfo,
                                                      const int32_t* __restrict__ rng, const void* __restrict__ xidx,
                                                      const TV* __restrict__ xv, const int32_t* __restrict__ cols,
-                                                     const T* __restrict__ p, T* __restrict__ q,
+                                                     const TX* __restrict__ p, T* __restrict__ q,
                                                      double* __restrict__ partial, int ccap,
                                                      const T* __restrict__ rdot, double* __restrict__ partial2) {
   extern __shared__ __attribute__((aligned(16))) char dsm[];
@@ -316,8 +318,8 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
   // list, all in the first burst: unconditional loads of clamped indices
   // (nr >= 1; the column list holds at least the block's own rows)
   const int tr = tid < nr ? tid : nr - 1;
-  const T pr = p[r0 + tr];
-  const T rr = (rdot ? rdot : p)[r0 + tr];
+  const T pr = (T)p[r0 + tr];
+  const T rr = rdot ? rdot[r0 + tr] : pr;
   // G threads per row in the row sums
   const int G = 4 * nr <= NT ? 4 : 2 * nr <= NT ? 2 : 1;
   const int grow = tid / G, gq = tid - grow * G;
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
   const i32x4* c4 = (const i32x4*)(cols + ci.x);
   const i32x4 cf = __builtin_nontemporal_load(c4 + (4 * tid < ci.y ? tid : 0));
   {
-    const T a0 = p[cf.x], a1 = p[cf.y], a2 = p[cf.z], a3 = p[cf.w];
+    const T a0 = (T)p[cf.x], a1 = (T)p[cf.y], a2 = (T)p[cf.z], a3 = (T)p[cf.w];
     if (4 * tid < ci.y) {
       xs[4 * tid] = a0;
       xs[4 * tid + 1] = a1;
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
   }
   for (int k = NT + tid; 4 * k < ci.y; k += NT) {
     const i32x4 c = __builtin_nontemporal_load(c4 + k);
-    const T a0 = p[c.x], a1 = p[c.y], a2 = p[c.z], a3 = p[c.w];
+    const T a0 = (T)p[c.x], a1 = (T)p[c.y], a2 = (T)p[c.z], a3 = (T)p[c.w];
     xs[4 * k] = a0;
     xs[4 * k + 1] = a1;
     xs[4 * k + 2] = a2;
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(NT) void k_spmv_blk_dot(const i32x4* __restrict__ d
       partial[blk] = contrib;
       partial2[blk] = c2;
     }
-  } else {
+  } else if (partial) {
     contrib = block_sum(contrib, sh);
     if (tid == 0) partial[blk] = contrib;
   }

@@ -1049,7 +1049,7 @@ void DevCsr::upload_pattern(const HostCsr& h, bool with_padded) {
   G4 = g > 64 ? 32 : g > 32 ? 16 : g > 12 ? 8 : 4;
 }
 
-void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
+void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr, bool with_f32) {
   n = h.n_rows;
   nnz = h.nnz();
   nb = (int)blk_ptr.size() - 1;
@@ -1150,7 +1150,7 @@ void BlkCsr::build(const HostCsr& h, const std::vector<int32_t>& blk_ptr) {
   d_pos.upload(pos.data(), 8 * pos.size());
   n_pad = std::max<int64_t>(4, n_ent);
   v64.alloc(8 * n_pad);
-  v32.alloc(4 * n_pad);
+  if (with_f32) v32.alloc(4 * n_pad);
   hip_check(hipMemset(v64.p, 0, 8 * n_pad), "memset");  // padding entries stay 0
 }
 
@@ -1166,8 +1166,9 @@ void BlkCsr::upload_values_device(const double* d_vals, hipStream_t st) {
   if (nb == 0) return;
   hipLaunchKernelGGL(k_scatter_map, dim3(grid_for(nnz)), dim3(BLOCK), 0, st, nnz, d_pos.as<int64_t>(), d_vals,
                      v64.as<double>());
-  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n_pad)), dim3(BLOCK), 0, st, n_pad,
-                     v64.as<double>(), v32.as<float>());
+  if (v32.p)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n_pad)), dim3(BLOCK), 0, st, n_pad,
+                       v64.as<double>(), v32.as<float>());
 }
 
 void DevCsr::derive_from_v64(hipStream_t st) {

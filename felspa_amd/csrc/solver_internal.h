@@ -87,7 +87,8 @@ struct BlkCsr {
   int nb = 0, ib = 1, cmax = 0, cnmax = 0, rmax = 0;
   DevBuf desc, cinfo, rng, idx, cols, v64, v32;
   int64_t n_pad = 0;  // entries of the padded layout
-  void build(const HostCsr& h, const std::vector<int32_t>& blk_ptr);
+  // with_f32: also an FP32 copy of the values
+  void build(const HostCsr& h, const std::vector<int32_t>& blk_ptr, bool with_f32 = true);
   // values in the plain layout, in device memory
   void upload_values_device(const double* d_vals, hipStream_t st);
   DevBuf d_pos;  // entry -> padded position

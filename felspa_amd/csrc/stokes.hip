@@ -236,6 +236,7 @@ struct fs_handle {
   hipStream_t st = nullptr;
   DevCsr K, Mp, Ablk;                                     // K, M_p and A alone, solver order
   BlkCsr Ablb;                                            // A by the ILU's lattice blocks (inner PCG SpMV)
+  BlkCsr Kblb;  // K's velocity rows (A and B^T) by the same blocks, FP64 (the outer method's K z)
   // value maps, caller entry -> K / Mp position and A entry -> K position:
   // built on the host at create, then kept on the device only
   std::vector<int64_t> mapA, mapBt, mapB, mapK11, mapMp, ablk_from_k;
@@ -689,6 +690,7 @@ struct fs_handle {
                          (const double*)K.v64.as<double>(), Ablk.v64.as<double>());
     Ablk.derive_from_v64(st);
     Ablb.upload_values_device(Ablk.v64.as<double>(), st);
+    Kblb.upload_values_device(K.v64.as<double>(), st);  // K's first rows, in K's entry order
     Mp.derive_from_v64(st);
     hip_check(hipGetLastError(), "device refactor");
     hip_check(hipStreamSynchronize(st), "sync");
@@ -774,18 +776,54 @@ struct fs_handle {
   // w = K~ z (TV matrix values, z in the preconditioned-vector precision T)
   template <typename TV, typename T>
   void kop_t(const T* z, double* w) {
-    if (!Dsc) {
-      spmv4_launch<TV, T, double, double, double, 0>(K, n, 0, z, w, (const double*)nullptr, 0, st);
+    const T* x = z;
+    if (Dsc) {
+      T* t = (T*)dsc_tT();
+      hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, z, Dsc, t, 1);
+      x = t;
+      cnt.launches++;
+      cnt.bytes += n * (2.0 * sizeof(T) + 8.0);
+    }
+    if (std::is_same<TV, double>::value && Kblb.nb > 0) {
+      // velocity rows by lattice blocks (2-byte column indices, the block's
+      // x values gathered once into LDS), pressure rows in CSR4
+      launch_kvel(x, w);
+      spmv4_launch<TV, T, double, double, double, 0>(K, np, nu, x, w + nu, (const double*)nullptr, 0, st);
+      cnt.launches += 2;
+      cnt.bytes += (double)Kblb.nnz * (8.0 + Kblb.ib) + 4.0 * (double)Kblb.n_cols + 4.0 * nu + 8.0 * nu +
+                   (K.nnz - (double)Kblb.nnz) * (sizeof(TV) + 4.0) + 8.0 * (np + 1) + n * sizeof(T) + 8.0 * np;
+    } else {
+      spmv4_launch<TV, T, double, double, double, 0>(K, n, 0, x, w, (const double*)nullptr, 0, st);
       cnt.launches++;
       cnt.bytes += K.nnz * (sizeof(TV) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0);
-      return;
     }
-    T* t = (T*)dsc_tT();
-    hipLaunchKernelGGL((k_mul_diag<T, T>), dim3(grid(n)), dim3(BLOCK), 0, st, n, z, Dsc, t, 1);
-    spmv4_launch<TV, T, double, double, double, 0>(K, n, 0, t, w, (const double*)nullptr, 0, st);
-    hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, w, Dsc, w, 1);
-    cnt.launches += 3;
-    cnt.bytes += K.nnz * (sizeof(TV) + 4.0) + 8.0 * (n + 1) + n * (sizeof(T) + 8.0) + n * (2.0 * sizeof(T) + 32.0);
+    if (Dsc) {
+      hipLaunchKernelGGL((k_mul_diag<double, double>), dim3(grid(n)), dim3(BLOCK), 0, st, n, w, Dsc, w, 1);
+      cnt.launches++;
+      cnt.bytes += 24.0 * n;
+    }
+  }
+  // w[0, nu) = (rows [0, nu) of K) x in FP64, x of type TX (all n entries)
+  template <typename TX>
+  void launch_kvel(const TX* x, double* w) {
+    const int ccap = (Kblb.cmax + 3) & ~3;
+    const size_t lds = sizeof(double) * ((size_t)ccap + Kblb.cnmax);
+#define FS_KB(IB, NT)                                                                                          \
+  {                                                                                                            \
+    auto fn = &k_spmv_blk_dot<double, double, IB, NT, TX>;                                                     \
+    if (lds > 64 * 1024)                                                                                       \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),    \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(Kblb.nb), dim3(NT), lds, st, Kblb.desc.as<i32x4>(), Kblb.cinfo.as<int2>(),     \
+                       Kblb.rng.as<int32_t>(), Kblb.idx.p, (const double*)Kblb.v64.p, Kblb.cols.as<int32_t>(), \
+                       x, w, (double*)nullptr, ccap, (const double*)nullptr, (double*)nullptr);                \
+  }
+    if (Kblb.rmax <= 64) {
+      if (Kblb.ib == 1) FS_KB(1, 128) else FS_KB(2, 128)
+    } else {
+      if (Kblb.ib == 1) FS_KB(1, 512) else FS_KB(2, 512)
+    }
+#undef FS_KB
   }
   // dst = P~^-1 src
   template <typename TV, typename T>
@@ -1058,6 +1096,16 @@ void build_K(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const fs_csr* B, c
   fs::phase_mark("A: device pattern + padded copy");
   h.Ablb.build(Ah, h.FA.sched.blk_ptr);
   fs::phase_mark("A by lattice blocks");
+  {
+    // K's velocity rows (all columns), for K z in FP64 by lattice blocks
+    HostCsr Kv;
+    Kv.n_rows = nu;
+    Kv.n_cols = n;
+    Kv.rp.assign(Kh.rp.begin(), Kh.rp.begin() + nu + 1);
+    Kv.ci.assign(Kh.ci.begin(), Kh.ci.begin() + Kh.rp[nu]);
+    h.Kblb.build(Kv, h.FA.sched.blk_ptr, false);
+  }
+  fs::phase_mark("K velocity rows by lattice blocks");
 }
 
 void fill_stats_err(fs_stats* s, int code) {
