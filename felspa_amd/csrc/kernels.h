@@ -1693,7 +1693,7 @@ __device__ __forceinline__ int node_probe(const uint32_t* tab, const int32_t* lc
 // columns and the hash map
 constexpr int UPD_BATCH = 64;
 __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) {
-  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 8 + 8 + 4 + 4 + 4) + (size_t)mcap * 4 +
+  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 8 + 8 + 4 + 4 + 4) + 16 + (size_t)mcap * 4 +
          (size_t)hs * 4;
 }
 
@@ -1718,11 +1718,14 @@ __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) 
 // the column-by-column IKJ loop; a multiplier is w_t times the pivot's
 // reciprocal (taken once per column and batch), within an ulp of w_t / u_kk.
 // Then the rest goes back to the factor.
+// The row group i .. i + G - 1 (same columns) from position pa of its rows
+// (m positions) in LDS, with the node-keyed hash of its columns; then its L
+// columns at positions [0, nc) applied in ascending order in node-triple
+// steps (sources: finished rows). The values stay in LDS (w, ld = mcap + 1)
+// for the caller; the multipliers of positions [0, nc) are already stored.
 template <int NT, int M, int G>
-__global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
-                                                           const uint8_t* __restrict__ tri, int nrows,
-                                                           int32_t col_a, int32_t col_b, int mcap, int hs) {
-  extern __shared__ __attribute__((aligned(16))) char smem[];
+__device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, const uint8_t* __restrict__ tri,
+                                                  int32_t i, int pa, int nc, int m, int mcap, int hs, char* smem) {
   double* w = (double*)smem;                     // [G][mcap + 1]; w[g][mcap]: dump slot
   int64_t* mq = (int64_t*)(w + G * (mcap + 1));  // [UPD_BATCH] first U entry of row k
   double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] 1 / u_kk
@@ -1731,25 +1734,14 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
   int32_t* mn = (int32_t*)(ms + UPD_BATCH);       // [UPD_BATCH] U entries of row k
   int32_t* tf = mn + UPD_BATCH;                   // [UPD_BATCH] slot starts a node triple
   int32_t* stp = tf + UPD_BATCH;                  // [UPD_BATCH] steps: slot | width << 8
-  int32_t* lc = stp + UPD_BATCH;                  // [mcap] columns
+  int32_t* nsteps = stp + UPD_BATCH;              // [4] the batch's step count
+  int32_t* lc = nsteps + 4;                       // [mcap] columns
   uint32_t* tab = (uint32_t*)(lc + mcap);         // [hs] hash map
-  __shared__ int n_steps;
   const uint32_t mask = (uint32_t)hs - 1;
   const int lg = __builtin_ctz((uint32_t)hs);
   const int tid = threadIdx.x, lane = tid & 63;
   const int ld = mcap + 1;
-  const int item = xcd_contiguous(blockIdx.x, gridDim.x);
-  if (item >= nrows) return;
-  const int32_t i = rows[item];
   const int64_t base = F.rp[i];
-  const int len = (int)(F.rp[i + 1] - base);
-  const int nx = F.o_lint[i];
-  // (every wave finds the same range)
-  const int pa = wave_lower_bound(F.ci + base, nx, col_a, lane);
-  const int pb = wave_lower_bound(F.ci + base, nx, col_b, lane);
-  if (pa == pb) return;  // the whole workgroup: no barrier is passed
-  const int m = len - pa;  // positions [pa, len): every column a U part of stage rows can name
-  const int nc = pb - pa;
   int64_t bg[G];  // the group's rows: row i + g starts at bg[g] + pa
 #pragma unroll
   for (int g = 0; g < G; ++g) bg[g] = F.rp[i + g] + pa;
@@ -1798,10 +1790,10 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
       const bool is_step = lane < nb && !mem;
       const uint64_t bal = __ballot(is_step);
       if (is_step) stp[__popcll(bal & ((1ull << lane) - 1ull))] = lane | ((st0 ? 3 : 1) << 8);
-      if (lane == 0) n_steps = __popcll(bal);
+      if (lane == 0) nsteps[0] = __popcll(bal);
     }
     lds_barrier();
-    const int ns = n_steps;
+    const int ns = nsteps[0];
     // The batch's steps as one stream of chunks of NT M entries, chunk
     // (c, off) = entries [off, off + NT M) of step c's U stream (a step with
     // an empty U stream still has its chunk: its multipliers). Loads run
@@ -1939,9 +1931,98 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
     }
     lds_barrier();  // the batch's last updates, before its metadata is replaced
   }
+}
+
+template <int NT, int M, int G>
+__global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, const int32_t* __restrict__ rows,
+                                                           const uint8_t* __restrict__ tri, int nrows,
+                                                           int32_t col_a, int32_t col_b, int mcap, int hs) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int item = xcd_contiguous(blockIdx.x, gridDim.x);
+  if (item >= nrows) return;
+  const int32_t i = rows[item];
+  const int64_t base = F.rp[i];
+  const int len = (int)(F.rp[i + 1] - base);
+  const int nx = F.o_lint[i];
+  // (every wave finds the same range)
+  const int pa = wave_lower_bound(F.ci + base, nx, col_a, lane);
+  const int pb = wave_lower_bound(F.ci + base, nx, col_b, lane);
+  if (pa == pb) return;  // the whole workgroup: no barrier is passed
+  const int m = len - pa;  // positions [pa, len): every column a U part of stage rows can name
+  const int nc = pb - pa;
+  ilu_group_columns<NT, M, G>(F, tri, i, pa, nc, m, mcap, hs, smem);
+  const double* w = (const double*)smem;
+  const int ld = mcap + 1;
 #pragma unroll
-  for (int g = 0; g < G; ++g)
-    for (int u = nc + tid; u < m; u += NT) F.v[bg[g] + u] = w[g * ld + u];
+  for (int g = 0; g < G; ++g) {
+    const int64_t bg = F.rp[i + g] + pa;
+    for (int u = nc + tid; u < m; u += NT) F.v[bg + u] = w[g * ld + u];
+  }
+}
+
+// In-block columns and pivots of a staged factorisation (its external
+// columns already applied): one workgroup per block of the stage, its row
+// groups in order, each a node triple (tri) or a single row. A group applies
+// its L columns of earlier rows of the block (ilu_group_columns, the sources
+// finished by this workgroup), then for a triple the in-node columns: row
+// r+1 eliminates column r, row r+2 columns r and r+1, each multiplier the
+// quotient by the pivot and each update one fused multiply-add per entry of
+// the pivot row's U part in LDS (the same order as the IKJ loop); then the
+// rows go to the factor, pivots checked.
+template <int NT, int M>
+__global__ __launch_bounds__(NT) void k_ilu_numeric_block(IluView<double> F, const uint8_t* __restrict__ tri,
+                                                          int blk0, int blk1, int mcap, int hs,
+                                                          int* __restrict__ err) {
+  extern __shared__ __attribute__((aligned(16))) char smem[];
+  const int tid = threadIdx.x;
+  const int blk = blk0 + blockIdx.x;
+  if (blk >= blk1) return;
+  const int r0 = F.blk_ptr[blk], r1 = F.blk_ptr[blk + 1];
+  const int ld = mcap + 1;
+  double* w = (double*)smem;
+  for (int r = r0; r < r1;) {
+    const int G = tri[r] ? 3 : 1;
+    const int64_t base = F.rp[r];
+    const int len = (int)(F.rp[r + 1] - base);
+    const int pa = F.o_lint[r];          // the first in-block column
+    const int nc = F.o_diag[r] - pa;     // the row's in-block L columns (rows r0 .. r-1)
+    const int m = len - pa;
+    if (G == 3) ilu_group_columns<NT, M, 3>(F, tri, r, pa, nc, m, mcap, hs, smem);
+    else ilu_group_columns<NT, M, 1>(F, tri, r, pa, nc, m, mcap, hs, smem);
+    if (G == 3) {
+      // rows r+1 and r+2 name columns r (position nc) and r + 1 (nc + 1)
+      const int d = nc;
+      const double m1 = w[ld + d] / w[d];
+      const double m2a = w[2 * ld + d] / w[d];
+      for (int u = d + 1 + tid; u < m; u += NT) {
+        const double x0 = w[u];
+        w[ld + u] = fma(-m1, x0, w[ld + u]);
+        w[2 * ld + u] = fma(-m2a, x0, w[2 * ld + u]);
+      }
+      lds_barrier();
+      const double m2b = w[2 * ld + d + 1] / w[ld + d + 1];
+      for (int u = d + 2 + tid; u < m; u += NT) w[2 * ld + u] = fma(-m2b, w[ld + u], w[2 * ld + u]);
+      lds_barrier();
+      if (tid == 0) {
+        w[ld + d] = m1;
+        w[2 * ld + d] = m2a;
+        w[2 * ld + d + 1] = m2b;
+      }
+      lds_barrier();
+    }
+    for (int g = 0; g < G; ++g) {
+      const int64_t bg = F.rp[r + g] + pa;
+      for (int u = nc + tid; u < m; u += NT) F.v[bg + u] = w[g * ld + u];
+      if (tid == 0) {
+        const double piv = w[g * ld + nc + g];
+        if (!(piv != 0.0) || !isfinite(piv)) atomicExch(err, 1);
+      }
+    }
+    __threadfence_block();
+    __syncthreads();  // the group's rows are sources of the block's later rows
+    r += G;
+  }
 }
 
 // ---------------------------------------------------------------------------

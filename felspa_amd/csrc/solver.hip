@@ -712,10 +712,32 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
         cnt.launches++;
       }
     }
+    const int ml = stage_max_len[s];
+    if (staged) {
+      // the in-block columns by node groups (k_ilu_numeric_block), when the
+      // stage's longest row with a node-keyed map of at most half load fits
+      int hsb = 64;
+      while (hsb < 2 * ml) hsb *= 2;
+      const size_t ldsb = ilu_update_lds_bytes(ml, hsb, 3);
+      if (ldsb <= dense_wg_lds_limit() && ml <= 8192) {
+#define FS_BLK(NT, M)                                                                                          \
+  {                                                                                                            \
+    auto fn = &k_ilu_numeric_block<NT, M>;                                                                     \
+    if (ldsb > 64 * 1024)                                                                                      \
+      hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb),   \
+                "hipFuncSetAttribute");                                                                        \
+    hipLaunchKernelGGL(fn, dim3(b1 - b0), dim3(NT), ldsb, st, V, tri.as<uint8_t>(), b0, b1, ml, hsb, d_err);    \
+  }
+        if (ml <= 1024) FS_BLK(256, 2)
+        else FS_BLK(768, 1)
+#undef FS_BLK
+        cnt.launches++;
+        continue;
+      }
+    }
     // LDS sized by the stage's longest row; threads x loads per thread cover
     // the longest U part its L columns can name (one wave up to 256 entries,
     // four up to 1024, sixteen waves of 2 loads each beyond)
-    const int ml = stage_max_len[s];
     int hs = 0, win = 0;
     const size_t lds = ilu_numeric_plan(ml, stage_max_nl[s], dense_wg_lds_limit(), hs, win);
     if (lds == 0) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the numeric kernel's LDS");
