@@ -249,10 +249,10 @@ def main():
     if args.kernels_only:
         kinds = {0: "A-block SpMV (k_spmv_blk_dot, FP32)",
                  2: "ILU(k) apply on A (k_ilu_ext_partials + k_ilu_stage_dense, FP32)",
-                 1: "K SpMV (k_spmv4, FP64)"}
+                 1: "K SpMV (FP64 values, the Arnoldi product)"}
         out = {"config_id": cfg, "kernels": {}}
         for kind in (0, 2, 1):
-            ms, by = lin.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED,
+            ms, by = lin.time_kernel(kind, precision=L.FS_PREC_MIXED,
                                      reps=args.kernel_reps)
             out["kernels"][kinds[kind]] = {"ms_per_unit": round(ms, 5), "bytes_per_unit_format": by,
                                            "reps": args.kernel_reps}
@@ -334,10 +334,10 @@ def main():
     fi = lin.factor_info(0)
     kinds = {0: "A-block SpMV (k_spmv_blk_dot, FP32)",
              2: "ILU(k) apply on A (k_ilu_ext_partials + k_ilu_stage_dense, FP32)",
-             1: "K SpMV (k_spmv4, FP64)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
+             1: "K SpMV (FP64 values, the Arnoldi product)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
     timing = {}
     for kind in (0, 2, 1, 3):
-        ms, by = lin.time_kernel(kind, precision=L.FS_PREC_FP64 if kind == 1 else L.FS_PREC_MIXED,
+        ms, by = lin.time_kernel(kind, precision=L.FS_PREC_MIXED,
                                  reps=args.kernel_reps)
         timing[kind] = (ms, by)
         log("kernel %-64s %.4f ms/unit  %.3e B  %.1f GB/s" % (kinds[kind], ms, by, by / ms / 1e6))
@@ -350,7 +350,7 @@ def main():
     ms_dom, by_fmt = timing[dom]
     nu = S.n_u
     if dom == 2:
-        # SURVEY 8(d) ILU apply bytes: nnz(ILU) (s_v + 4) + 2 (m+1) 8 + 3 m s_x, FP32 values and vectors
+        # ILU apply bytes: nnz(ILU) (s_v + 4) + 2 (m+1) 8 + 3 m s_x, FP32 values and vectors
         by_alg = fi["nnz"] * (4 + 4) + 2 * (nu + 1) * 8 + 3 * nu * 4
     else:
         # CSR SpMV: nnz (s_v + 4) + (m+1) 8 + 2 m s_x

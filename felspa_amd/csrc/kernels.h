@@ -1969,8 +1969,11 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_update(IluView<double> F, co
 // r+1 eliminates column r, row r+2 columns r and r+1, each multiplier the
 // quotient by the pivot and each update one fused multiply-add per entry of
 // the pivot row's U part in LDS (the same order as the IKJ loop); then the
-// rows go to the factor, pivots checked.
-template <int NT, int M>
+// rows go to the factor, pivots checked. Without TRI (a stage whose three
+// rows do not fit in LDS together) every row is its own group: a triple's
+// later rows then take its in-node columns as ordinary L columns of finished
+// rows, in ascending order as in the IKJ loop.
+template <int NT, int M, bool TRI>
 __global__ __launch_bounds__(NT) void k_ilu_numeric_block(IluView<double> F, const uint8_t* __restrict__ tri,
                                                           int blk0, int blk1, int mcap, int hs,
                                                           int* __restrict__ err) {
@@ -1982,7 +1985,7 @@ __global__ __launch_bounds__(NT) void k_ilu_numeric_block(IluView<double> F, con
   const int ld = mcap + 1;
   double* w = (double*)smem;
   for (int r = r0; r < r1;) {
-    const int G = tri[r] ? 3 : 1;
+    const int G = TRI && tri[r] ? 3 : 1;
     const int64_t base = F.rp[r];
     const int len = (int)(F.rp[r + 1] - base);
     const int pa = F.o_lint[r];          // the first in-block column

@@ -668,6 +668,11 @@ static int grid_for(int64_t n, int per_block = BLOCK, int cap = 4096) {
 void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv, int64_t col_limit,
                        hipStream_t st, int* d_err, Counters& cnt) {
   IluView<double> V = view<double>();
+  // FS_ILU_ROWWISE=1 runs every in-block stage one row at a time (the form
+  // taken when three rows of a stage do not fit in LDS), e.g. to test it on
+  // small systems
+  const char* rw = std::getenv("FS_ILU_ROWWISE");
+  const bool rowwise = rw && std::atoi(rw) == 1;
   if (staged) {
     hip_check(hipMemsetAsync(v64.p, 0, 8 * (size_t)nnz, st), "memset");
     hipLaunchKernelGGL(k_ilu_source_scatter, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, st, V, n, srp, sci, sv,
@@ -715,14 +720,21 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
     const int ml = stage_max_len[s];
     if (staged) {
       // the in-block columns by node groups (k_ilu_numeric_block), when the
-      // stage's longest row with a node-keyed map of at most half load fits
+      // stage's longest row fits with a node-keyed map of at most half load,
+      // three rows at once or else one (a row names at most ml nodes, so a
+      // map of more than ml slots is the last resort)
       int hsb = 64;
       while (hsb < 2 * ml) hsb *= 2;
-      const size_t ldsb = ilu_update_lds_bytes(ml, hsb, 3);
+      const bool tri3 = ilu_update_lds_bytes(ml, hsb, 3) <= dense_wg_lds_limit() && !rowwise;
+      if (!tri3 && ilu_update_lds_bytes(ml, hsb, 1) > dense_wg_lds_limit()) {
+        hsb = 64;
+        while (hsb < ml + 1) hsb *= 2;
+      }
+      const size_t ldsb = ilu_update_lds_bytes(ml, hsb, tri3 ? 3 : 1);
       if (ldsb <= dense_wg_lds_limit() && ml <= 8192) {
 #define FS_BLK(NT, M)                                                                                          \
   {                                                                                                            \
-    auto fn = &k_ilu_numeric_block<NT, M>;                                                                     \
+    auto fn = tri3 ? &k_ilu_numeric_block<NT, M, true> : &k_ilu_numeric_block<NT, M, false>;                   \
     if (ldsb > 64 * 1024)                                                                                      \
       hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsb),   \
                 "hipFuncSetAttribute");                                                                        \

@@ -1731,8 +1731,9 @@ int fs_stokes_time_kernel(fs_handle* h, int32_t kind, int32_t precision, int32_t
         h->cnt.bytes = h->ablk_spmv_bytes(sv, sx, true);
         break;
       }
-      case 1:
-        h->kmult64(h->vin.as<double>(), h->r.as<double>(), nullptr);
+      case 1:  // the Arnoldi product w = K z with FP64 values
+        if (f32) h->kop_t<double, float>((const float*)h->ytmp.p, h->r.as<double>());
+        else h->kop_t<double, double>(h->vin.as<double>(), h->r.as<double>());
         break;
       case 2:
         if (f32) h->time_ilu_step<float, float>((float*)upd.p, (float*)base, (float*)base + h->ldu);

@@ -237,8 +237,9 @@ int fs_stokes_get_factor_info(fs_handle* h, int32_t which, int64_t* n, int64_t* 
  * class on the handle's own stream between HIP events, with the current
  * factorisation and solver-order data, and returns the mean duration of one
  * launch (ms) and its algorithmic HBM bytes. kind: 0 = A-block SpMV of the
- * inner PCG (q = A p with the p.q and r.p partials), 1 = K SpMV (FP64 true
- * residual), 2 = one ILU(k) application on A as the inner PCG runs it (all
+ * inner PCG (q = A p with the p.q and r.p partials), 1 = K SpMV with FP64
+ * values as the outer Arnoldi step runs it (the input vector in FP32 for
+ * FS_PREC_MIXED, FP64 otherwise), 2 = one ILU(k) application on A as the inner PCG runs it (all
  * stage launches, the forward stages carrying the PCG update step), 3 = one
  * inner CG on M_p, 4 = one numeric ILU(k) factorisation of A (FP64, all its
  * launches and the block inverses; precision ignored). */

@@ -158,6 +158,22 @@ def test_ilu2_block_color(oracle, cfg, n, hint):
         assert np.abs(F2.data - R2.data).max() <= 1e-12 * np.abs(R2.data).max()
 
 
+@pytest.mark.parametrize("cfg,n,hint", [("C5", 6, 3), ("C3", 4, 3)])
+def test_ilu2_rowwise_in_block_stages(oracle, monkeypatch, cfg, n, hint):
+    """The in-block stages one row at a time (the form long-row stages take
+    when three rows do not fit in LDS, forced by FS_ILU_ROWWISE=1): the
+    factor is the reference SparseILU of P A P^T to 1e-12."""
+    monkeypatch.setenv("FS_ILU_ROWWISE", "1")
+    S = system(cfg, n)
+    A = S.blocks["A"]
+    ilu = fa.SparseILU(ordering=L.FS_ORDER_BLOCK_COLOR, block_hint=S.block_hint(hint))
+    ilu.initialize(A, fa.SparseILU.AdditionalData(level_of_fill=2))
+    F, perm = ilu.factors()
+    R = oracle.SparseILU(_perm_matrix(A, perm), 2).factors()
+    assert (F.indptr == R.indptr).all() and (F.indices == R.indices).all()
+    assert np.abs(F.data - R.data).max() <= 1e-12 * np.abs(R.data).max()
+
+
 def test_ilu_oversize_row_unsupported():
     """A factor row longer than the numeric kernel's LDS plan (an arrow
     matrix: the last row and column full) is refused at the symbolic phase
