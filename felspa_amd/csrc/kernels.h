@@ -1912,17 +1912,21 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
     issue(p0, ja, va);
     issue(p1, jb, vb);
     issue(p2, jc, vc);
-    for (;;) {
-      if (p0.x >= ns) break;
+    // One exit, at the loop test: every pass issues the three chunks' loads
+    // (past the batch, loads of entry 0), so the loads in flight when a
+    // chunk is used are the same on every path into it and the compiler's
+    // memory-counter waits keep the other two chunks in flight (with early
+    // exits in the body, a shared exit block made them wait for all loads).
+    // Positions only advance, so a chunk past the batch is followed by
+    // chunks past it.
+    while (p0.x < ns) {
       process(p0, ja, va);
       const int2 p3 = adv(p2);
       issue(p3, ja, va);
-      if (p1.x >= ns) break;
-      process(p1, jb, vb);
+      if (p1.x < ns) process(p1, jb, vb);
       const int2 p4 = adv(p3);
       issue(p4, jb, vb);
-      if (p2.x >= ns) break;
-      process(p2, jc, vc);
+      if (p2.x < ns) process(p2, jc, vc);
       const int2 p5 = adv(p4);
       issue(p5, jc, vc);
       p0 = p3;
