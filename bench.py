@@ -203,12 +203,14 @@ def main():
     restart = args.restart or settings["restart"]
     t0 = time.time()
     # one handle: the mixed and the plain FP64 solves share the symbolic phase
-    # and the factor (the precision is switched between solves); the FP64
-    # method's inner CG on M_p stops at the same relative 1e-2 / 50 steps
+    # and the factor (the precision is switched between solves); both stop the
+    # inner CG on M_p at the configuration's relative tolerance and step cap
+    cg_tol, cg_max = settings["inner_cg_tol"], settings["inner_cg_max"]
     lin = fa.StokesLinearSystem(S.blocks, A_lof=kA, S_lof=kS, precision=L.FS_PREC_MIXED, guess=L.FS_GUESS_ZERO,
                                 block_hint=S.block_hint(hint_nodes), constraints=S.constrained,
                                 restart=restart, inner_a_iters=inner_a, max_outer=3000,
-                                inner_cg_tol=1e-2, inner_cg_max=50)
+                                inner_cg_tol=cg_tol, inner_cg_max=cg_max, inner_cg_tol_mixed=cg_tol,
+                                inner_cg_max_mixed=cg_max)
     t_create = time.time() - t0
     log("handle: orderings + symbolic ILU(%d) + upload %.1fs" % (kA, t_create))
     model, weights = None, None
@@ -334,7 +336,7 @@ def main():
     fi = lin.factor_info(0)
     kinds = {0: "A-block SpMV (k_spmv_blk_dot, FP32)",
              2: "ILU(k) apply on A (k_ilu_ext_partials + k_ilu_stage_dense, FP32)",
-             1: "K SpMV (FP64 values, the Arnoldi product)", 3: "inner CG on M_p (k_inner_cg, FP32)"}
+             1: "K SpMV (FP64 values, the Arnoldi product)", 3: "inner CG on M_p (FP32)"}
     timing = {}
     for kind in (0, 2, 1, 3):
         ms, by = lin.time_kernel(kind, precision=L.FS_PREC_MIXED,
@@ -409,7 +411,8 @@ def main():
         "data": "synthetic (generated Q2-Q1 system, seeded field)",
         "config": {"workload": WORKLOADS.get(cfg, cfg), "config_id": cfg, "sample": sample,
                    "n_u": S.n_u, "n_p": S.n_p, "nnz_A": int(S.blocks["A"].nnz), "ilu_k_A": kA, "ilu_k_S": kS,
-                   "restart": restart, "inner_a_pcg": inner_a, "block_nodes": hint_nodes,
+                   "restart": restart, "inner_a_pcg": inner_a, "inner_cg_tol": cg_tol, "inner_cg_max": cg_max,
+                   "block_nodes": hint_nodes,
                    "guess": "learned" if model is not None else "zero", "learned_guess": model is not None,
                    "weights": os.path.relpath(weights, ROOT) if model is not None else None},
         "outer_iters": st_head["outer_iters"], "inner_cg_iters": st_head["inner_iters"],

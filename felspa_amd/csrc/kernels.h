@@ -2217,6 +2217,96 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   return tot;
 }
 
+// ---------------------------------------------------------------------------
+// The inner CG on M_p capped at one step (k_inner_cg's arithmetic with
+// max_it = 1) as ordinary launches, without grid barriers: from x = 0 the
+// residual is g = -b and the first direction d = -ILU_S^{-1} g = ILU_S^{-1} b
+// (the factor's stage kernels), then
+//   k_cg_step_spmv: q = M d and per-workgroup partials of d.q, b.d, b.b
+//   k_cg_step_x:    alpha = b.d / d.q (= g.h / d.q), x = alpha d and partials
+//                   of |g + alpha q|^2 with g = -b
+//   k_cg_step_end:  the step count and the convergence flag
+// Each kernel sums the previous one's partials in the same fixed order in
+// every workgroup. At most CG_STEP_WG workgroups per kernel (grid-stride).
+constexpr int CG_STEP_WG = 512;
+
+template <typename T, int G>
+__global__ __launch_bounds__(BLOCK) void k_cg_step_spmv(int64_t n, const int64_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci, const T* __restrict__ val,
+                                                        const T* __restrict__ b, const T* __restrict__ d,
+                                                        T* __restrict__ q, double* __restrict__ part) {
+  __shared__ double sh[BLOCK / WAVE];
+  const int lane = threadIdx.x % G;
+  double dq = 0, bd = 0, bb = 0;
+  for (int64_t row = (int64_t)blockIdx.x * (BLOCK / G) + threadIdx.x / G; row < n;
+       row += (int64_t)gridDim.x * (BLOCK / G)) {
+    const T acc = row_dot<T, T, T, G, false>(ci, val, d, rp[row], rp[row + 1], lane, 0, 0);
+    if (lane == 0) {
+      q[row] = acc;
+      const double dr = (double)d[row], br = (double)b[row];
+      dq += (double)acc * dr;
+      bd += br * dr;
+      bb += br * br;
+    }
+  }
+  dq = block_sum(dq, sh);
+  bd = block_sum(bd, sh);
+  bb = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = dq;
+    part[CG_STEP_WG + blockIdx.x] = bd;
+    part[2 * CG_STEP_WG + blockIdx.x] = bb;
+  }
+}
+
+// the step's start: |b| and whether the CG stops before its first step
+// (res <= reduce * res, i.e. b = 0 for reduce < 1, or res <= tol)
+__device__ __forceinline__ bool cg_step_done0(const double* part, int nb1, double tol, double reduce, double* sh,
+                                              double& res0) {
+  res0 = sqrt(sum_partials(part + 2 * CG_STEP_WG, nb1, sh));
+  return res0 <= res0 * reduce || res0 <= tol;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_cg_step_x(int64_t n, int nb1, const T* __restrict__ b,
+                                                     const T* __restrict__ d, const T* __restrict__ q,
+                                                     T* __restrict__ x, double* __restrict__ part, double tol,
+                                                     double reduce) {
+  __shared__ double sh[BLOCK / WAVE];
+  double res0;
+  const bool done0 = cg_step_done0(part, nb1, tol, reduce, sh, res0);
+  const double dq = sum_partials(part, nb1, sh);
+  const double bd = sum_partials(part + CG_STEP_WG, nb1, sh);
+  const double alpha = bd / dq;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    if (done0) {
+      x[i] = 0;
+    } else {
+      x[i] = (T)alpha * d[i];
+      const T gi = -b[i] + (T)alpha * q[i];
+      s += (double)gi * (double)gi;
+    }
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[3 * CG_STEP_WG + blockIdx.x] = s;
+}
+
+// one workgroup: iterations += 1 unless the CG stopped before its step; the
+// failure flag when the step did not reach the tolerance
+static __global__ __launch_bounds__(BLOCK) void k_cg_step_end(int nb1, int nb2, const double* __restrict__ part,
+                                                              double tol, double reduce, int* __restrict__ iters,
+                                                              int* __restrict__ fail) {
+  __shared__ double sh[BLOCK / WAVE];
+  double res0;
+  const bool done0 = cg_step_done0(part, nb1, tol, reduce, sh, res0);
+  const double res = sqrt(sum_partials(part + 3 * CG_STEP_WG, nb2, sh));
+  if (threadIdx.x == 0 && !done0) {
+    iters[0] += 1;
+    if (!(res <= res0 * reduce || res <= tol)) fail[0] = 1;
+  }
+}
+
 // partial[b] = sum a_i b_i
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_dot_partial_t(int64_t n, const T* __restrict__ a, const T* __restrict__ b,

@@ -339,9 +339,13 @@ struct fs_handle {
     a.fail_out = dptr<int>(flags, 2);
     a.gb = dptr<GridBarrier>(gbar);
     a.timeout = timeout_ticks();
+    const int gm = Mp.G <= 8 ? 8 : Mp.G <= 16 ? 16 : 32;
+    if (a.max_it == 1) {
+      inner_cg_step<T>(a, gm);
+      return;
+    }
     const size_t lds = lds_bytes_per_wave(FS.max_rows, FS.cap, sizeof(T));
     void* fn;
-    const int gm = Mp.G <= 8 ? 8 : Mp.G <= 16 ? 16 : 32;
     fn = pick_cg_kernel<T>(gm, FS.G);
     int per_cu = 0;
     hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLOCK, lds), "occupancy");
@@ -353,6 +357,31 @@ struct fs_handle {
               "inner CG cooperative launch");
     cnt.launches++;
     cnt.bytes += (Mp.nnz + FS.nnz) * (sizeof(T) + 4.0) + 8.0 * np * sizeof(T);
+  }
+
+  // The inner CG capped at one step, by ordinary launches: the cooperative
+  // kernel's grid barriers (two per colour stage of ILU_S and one per
+  // reduction) cost more than the step's work. Same arithmetic: d = ILU_S^{-1}
+  // b, q = M d, alpha = b.d / d.q, x = alpha d; the step count and the
+  // failure flag as the cooperative kernel sets them.
+  template <typename T>
+  void inner_cg_step(const InnerCgArgs<T>& a, int gm) {
+    FS.apply<T, T, T>(a.b, a.h, a.d, st, cnt);
+    double* part = dptr<double>(partial);
+    const int rows = BLOCK / gm;
+    const int nb1 = (int)std::min<int64_t>(CG_STEP_WG, std::max<int64_t>(1, (np + rows - 1) / rows));
+    const int nb2 = (int)std::min<int64_t>(CG_STEP_WG, std::max<int64_t>(1, (np + BLOCK - 1) / BLOCK));
+#define FS_CGS(GM)                                                                                            \
+  hipLaunchKernelGGL((k_cg_step_spmv<T, GM>), dim3(nb1), dim3(BLOCK), 0, st, np, a.mrp, a.mci, a.mv, a.b, a.d, \
+                     a.q, part);
+    if (gm == 8) FS_CGS(8) else if (gm == 16) FS_CGS(16) else FS_CGS(32)
+#undef FS_CGS
+    hipLaunchKernelGGL((k_cg_step_x<T>), dim3(nb2), dim3(BLOCK), 0, st, np, nb1, a.b, a.d, a.q, a.x, part, a.tol,
+                       a.reduce);
+    hipLaunchKernelGGL(k_cg_step_end, dim3(1), dim3(BLOCK), 0, st, nb1, nb2, (const double*)part, a.tol, a.reduce,
+                       a.iters_out, a.fail_out);
+    cnt.launches += 3;
+    cnt.bytes += Mp.nnz * (sizeof(T) + 4.0) + 8.0 * (np + 1) + 6.0 * np * sizeof(T);
   }
 
   template <typename T>

@@ -157,6 +157,47 @@ def test_fgmres_tight_tolerance(cfg, n, kA, precision):
         assert ep < 1e-4, ep
 
 
+def test_inner_cg_one_step_by_launches(oracle):
+    """The inner CG on M_p capped at one step runs as ordinary launches: the
+    block preconditioner's pressure part is one ILU(0)-PCG step from zero,
+    x = (t.d / d.M d) d with d = ILU_S^{-1} t and t = B ILU_A^{-1} src_u -
+    src_p (the oracle's factors, reference order), equal to the cooperative
+    kernel's result when that one stops after its first step (cap 2), in
+    FP64 and with FP32 vectors; a step short of the tolerance is reported as
+    FS_NOT_CONVERGED."""
+    S = system("C1", 8)
+    nu = S.n_u
+    src = np.random.default_rng(9).standard_normal(S.n)
+    du = oracle.SparseILU(S.blocks["A"], 1).vmult(src[:nu])
+    t = S.blocks["B"] @ du - src[nu:]
+    d = oracle.SparseILU(S.blocks["Mp"], 0).vmult(t)
+    Md = S.blocks["Mp"] @ d
+    alpha = (t @ d) / (d @ Md)
+    want_p = alpha * d
+    # the premise: one step reduces the residual below 0.95 |t|
+    assert np.linalg.norm(alpha * Md - t) < 0.95 * np.linalg.norm(t)
+    out = {}
+    for prec, tol_p in ((L.FS_PREC_FP64, 1e-12), (L.FS_PREC_MIXED, 1e-5)):
+        for cap in (1, 2):
+            lin = fa.StokesLinearSystem(S.blocks, A_lof=1, S_lof=0, ordering=L.FS_ORDER_REFERENCE,
+                                        method=L.FS_METHOD_FGMRES, precision=prec, inner_a_iters=0,
+                                        inner_cg_tol=0.95, inner_cg_max=cap, inner_cg_tol_mixed=0.95,
+                                        inner_cg_max_mixed=cap)
+            dst, st = lin.precondition(src, precision=prec)
+            assert st["status"] == L.FS_OK and st["inner_iters"] == 1
+            assert rel(dst[:nu], du) < tol_p
+            out[prec, cap] = dst[nu:]
+            lin.close()
+        assert rel(out[prec, 1], want_p) < tol_p
+        assert rel(out[prec, 1], out[prec, 2]) < tol_p
+    lin = fa.StokesLinearSystem(S.blocks, A_lof=1, S_lof=0, ordering=L.FS_ORDER_REFERENCE,
+                                method=L.FS_METHOD_FGMRES, inner_a_iters=0, inner_cg_tol=1e-8, inner_cg_max=1)
+    dst, st = lin.precondition(src)
+    assert st["status"] == L.FS_NOT_CONVERGED and st["inner_iters"] == 1
+    assert rel(dst[nu:], want_p) < 1e-12
+    lin.close()
+
+
 def test_refactor_and_warm_start():
     """New values on the same pattern; a converged guess needs no iteration."""
     S = system("C2", 8)
