@@ -13,6 +13,7 @@ constexpr int BLOCK = 256;      // threads per workgroup for streaming kernels
 constexpr int RED_BLOCKS = 512; // partial-sum slots of the two-stage reductions
 constexpr int MAXM = 256;       // max vectors in one multi-dot (restart + 1)
 constexpr int MDOT_CHUNK = 32;  // vectors per workgroup row of a multi-dot
+constexpr int MDOT2_CHUNK = 32; // basis vectors per workgroup row of the two-vector multi-dot
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -535,6 +536,209 @@ __global__ __launch_bounds__(BLOCK) void k_gs_update(int64_t n, int m, const dou
     s = block_sum(s, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Arnoldi step with delayed reorthogonalisation (DCGS2: Bielich, Langou,
+// Thomas, Swirydowicz, Yamazaki, Boman, "Low-synch Gram-Schmidt with delayed
+// reorthogonalization for Krylov solvers", Parallel Computing 2022). At step
+// j the basis column j holds u_j, the previous step's vector after one
+// projection pass, scaled by that pass's norm nu_{j-1}; columns 0..j-1 are
+// final. One pass over V gives both a = V_{<j}^T u_j, alpha = u_j.u_j (the
+// reorthogonalisation of u_j) and t = V_{<j}^T w, tau = u_j.w (the projection
+// of the new vector w); then v_j = (u_j - V_{<j} a) / r with r^2 = alpha -
+// a.a, v_j.w = (tau - a.t) / r, and one more pass writes v_j and the
+// projected w. The previous Hessenberg column gets nu_{j-1} a added and
+// h_{j,j-1} = nu_{j-1} r.
+
+// partial[b*MAXM + k] = V_k . x1, partial2[b*MAXM + k] = V_k . x2 (k < m):
+// k_mdot_partial for two vectors, V read once
+template <int CH>
+__global__ __launch_bounds__(BLOCK) void k_mdot2_partial(int64_t n, int m, const double* __restrict__ V, int64_t ld,
+                                                         const double* __restrict__ x1,
+                                                         const double* __restrict__ x2,
+                                                         double* __restrict__ partial,
+                                                         double* __restrict__ partial2) {
+  __shared__ double sh[2 * CH][BLOCK / WAVE];
+  const int k0 = blockIdx.y * CH;
+  const int mc = min(CH, m - k0);
+  const double* Vc = V + (int64_t)k0 * ld;
+  double a1[CH], a2[CH];
+#pragma unroll
+  for (int k = 0; k < CH; ++k) a1[k] = a2[k] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    const double y1 = x1[i], y2 = x2[i];
+#pragma unroll
+    for (int k = 0; k < CH; ++k)
+      if (k < mc) {
+        const double v = Vc[(int64_t)k * ld + i];
+        a1[k] += v * y1;
+        a2[k] += v * y2;
+      }
+  }
+  const int wv = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    if (k < mc) {
+      const double s1 = group_reduce<WAVE>(a1[k]);
+      const double s2 = group_reduce<WAVE>(a2[k]);
+      if (l == 0) {
+        sh[k][wv] = s1;
+        sh[CH + k][wv] = s2;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)(2 * mc)) {
+    const int k = threadIdx.x < (unsigned)mc ? threadIdx.x : threadIdx.x - mc;
+    const int row = threadIdx.x < (unsigned)mc ? k : CH + k;
+    double s = 0;
+    for (int i = 0; i < BLOCK / WAVE; ++i) s += sh[row][i];
+    (threadIdx.x < (unsigned)mc ? partial : partial2)[(int64_t)blockIdx.x * MAXM + k0 + k] = s;
+  }
+}
+
+// out[k] = sum_b partial[b*MAXM + k] (blocks k < m), out2[k] from partial2
+// (blocks m + k)
+static __global__ void k_mdot2_final(int nb, int m, const double* __restrict__ partial,
+                                     const double* __restrict__ partial2, double* __restrict__ out,
+                                     double* __restrict__ out2) {
+  const bool second = (int)blockIdx.x >= m;
+  const int k = second ? blockIdx.x - m : blockIdx.x;
+  const double* p = second ? partial2 : partial;
+  __shared__ double sh[BLOCK / WAVE];
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += p[(int64_t)i * MAXM + k];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) (second ? out2 : out)[k] = s;
+}
+
+// One thread: from a, alpha (h1[0..j]) and t, tau (h2[0..j]) the step's
+// 1 / r and v_j.w (dc[0], dc[1]); the previous column corrected into raw0
+// (rows 0..j, from the raw column j - 1 in raw1 and nu_{j-1} = nrm[0]) and
+// the new raw column j (rows 0..j) into raw1. u_0 = v_0 is final: r = 1.
+static __global__ void k_dcgs2_coef(int j, const double* __restrict__ h1, const double* __restrict__ h2,
+                                    const double* __restrict__ nrm, double* __restrict__ raw0,
+                                    double* __restrict__ raw1, double* __restrict__ dc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (j == 0) {
+    dc[0] = 1.0;
+    dc[1] = h2[0];
+    raw1[0] = h2[0];
+    return;
+  }
+  double aa = 0, at = 0;
+  for (int i = 0; i < j; ++i) {
+    aa += h1[i] * h1[i];
+    at += h1[i] * h2[i];
+  }
+  // r^2 <= 0: u_j lies in the span of the basis (a breakdown the projection
+  // pass did not see); v_j = 0 then
+  const double r2 = h1[j] - aa;
+  const double r = r2 > 0.0 ? sqrt(r2) : 0.0;
+  const double rinv = r > 0.0 ? 1.0 / r : 0.0;
+  const double hjj = (h2[j] - at) * rinv;
+  const double nu = nrm[0];
+  for (int i = 0; i < j; ++i) raw0[i] = raw1[i] + nu * h1[i];
+  raw0[j] = nu * r;
+  for (int i = 0; i < j; ++i) raw1[i] = h2[i];
+  raw1[j] = hjj;
+  dc[0] = rinv;
+  dc[1] = hjj;
+}
+
+// v_j = (u_j - sum_{k<j} a_k V_k) / r into column j (j > 0) and
+// w -= sum_{k<j} t_k V_k + (v_j.w) v_j; partial[b] = sum_i w_i^2
+static __global__ __launch_bounds__(BLOCK) void k_dcgs2_update(int64_t n, int j, double* __restrict__ V, int64_t ld,
+                                                               const double* __restrict__ a,
+                                                               const double* __restrict__ t,
+                                                               const double* __restrict__ dc,
+                                                               double* __restrict__ w,
+                                                               double* __restrict__ partial) {
+  __shared__ double sh[BLOCK / WAVE];
+  __shared__ double ak[MAXM], tk[MAXM];
+  for (int k = threadIdx.x; k < j; k += BLOCK) {
+    ak[k] = a[k];
+    tk[k] = t[k];
+  }
+  __syncthreads();
+  const double rinv = dc[0], hjj = dc[1];
+  double* u = V + (int64_t)j * ld;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    double sa = 0, st = 0;
+    int k = 0;
+    for (; k + 8 <= j; k += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = V[(int64_t)(k + q) * ld + i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        sa += ak[k + q] * v[q];
+        st += tk[k + q] * v[q];
+      }
+    }
+    for (; k < j; ++k) {
+      const double v = V[(int64_t)k * ld + i];
+      sa += ak[k] * v;
+      st += tk[k] * v;
+    }
+    double vj = u[i];
+    if (j > 0) {
+      vj = (vj - sa) * rinv;
+      u[i] = vj;
+    }
+    const double wi = w[i] - st - hjj * vj;
+    w[i] = wi;
+    s += wi * wi;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// One workgroup: nu_j = |w| from the update's partials (nrm[0]); then
+// thread 0 commits column j - 1 (its corrected raw column raw0, rows 0..j:
+// rotations 0..j-2, its own rotation, gamma from gsave = gamma_{j-1} before
+// that rotation) and makes column j provisional (raw1 rows 0..j and h_{j+1,j}
+// = nu_j, the norm before u_{j+1}'s reorthogonalisation), saving gamma_j
+// first; est = |gamma_{j+1}|. A cycle's last column stays provisional.
+static __global__ void k_givens_lag(int j, int ldh, int nb, const double* __restrict__ partial,
+                                    const double* __restrict__ raw0, const double* __restrict__ raw1,
+                                    double* __restrict__ hcol, double* __restrict__ H, double* __restrict__ cs,
+                                    double* __restrict__ sn, double* __restrict__ gam, double* __restrict__ gsave,
+                                    double* __restrict__ nrm, double* __restrict__ est) {
+  __shared__ double sh[BLOCK / WAVE];
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x != 0) return;
+  const double nu = sqrt(s);
+  nrm[0] = nu;
+  auto rotate = [&](int c) {  // column c in hcol[0..c+1]
+    for (int i = 0; i < c; ++i) {
+      const double hi = hcol[i];
+      hcol[i] = cs[i] * hi + sn[i] * hcol[i + 1];
+      hcol[i + 1] = -sn[i] * hi + cs[i] * hcol[i + 1];
+    }
+    const double r = 1.0 / sqrt(hcol[c] * hcol[c] + hcol[c + 1] * hcol[c + 1]);
+    sn[c] = hcol[c + 1] * r;
+    cs[c] = hcol[c] * r;
+    hcol[c] = cs[c] * hcol[c] + sn[c] * hcol[c + 1];
+    for (int i = 0; i <= c; ++i) H[(int64_t)i * ldh + c] = hcol[i];
+  };
+  if (j > 0) {
+    for (int i = 0; i <= j; ++i) hcol[i] = raw0[i];
+    rotate(j - 1);
+    gam[j] = -sn[j - 1] * gsave[0];
+    gam[j - 1] = cs[j - 1] * gsave[0];
+  }
+  for (int i = 0; i <= j; ++i) hcol[i] = raw1[i];
+  hcol[j + 1] = nu;
+  rotate(j);
+  gsave[0] = gam[j];
+  gam[j + 1] = -sn[j] * gam[j];
+  gam[j] *= cs[j];
+  est[0] = fabs(gam[j + 1]);
 }
 
 // dst = src / scal[idx] (0 when the norm is zero: lucky breakdown)

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -45,7 +46,10 @@ constexpr int SC_NRM = SC_Y + MAXM;         // 1
 constexpr int SC_EST = SC_NRM + 1;          // 1
 constexpr int SC_TMP = SC_EST + 1;          // 4
 constexpr int SC_PCG = SC_TMP + 4;          // 4: p.q, r.z (two slots)
-constexpr int SC_H = SC_PCG + 4;            // [(MAXM+1) * MAXM]
+constexpr int SC_DC = SC_PCG + 4;           // 4: DCGS2 1 / r, v_j.w, saved gamma_j
+constexpr int SC_RAW0 = SC_DC + 4;          // [MAXM+1] DCGS2 corrected previous column
+constexpr int SC_RAW1 = SC_RAW0 + MAXM + 1; // [MAXM+1] DCGS2 raw current column
+constexpr int SC_H = SC_RAW1 + MAXM + 1;    // [(MAXM+1) * MAXM]
 constexpr int SC_TOTAL = SC_H + (MAXM + 1) * MAXM;
 
 struct Ctrl {  // dealii::ReductionControl::check (documented semantics)
@@ -539,6 +543,38 @@ struct fs_handle {
     cnt.bytes += 8.0 * n * (2.0 * (m + 1) + 2.0 * (m + 2) + 2.0);
   }
 
+  // The flexible method's Arnoldi step with delayed reorthogonalisation
+  // (DCGS2): two passes over the basis instead of classical Gram-Schmidt's
+  // four (two projections, two updates). Column j holds u_j (one projection
+  // pass, scaled by its norm); one pass forms V^T u_j and V^T w, the next
+  // writes the final v_j and the projected w; the previous Hessenberg column
+  // is corrected and its Givens rotation committed, column j's made
+  // provisional with h_{j+1,j} = |w| for the residual estimate.
+  void arnoldi_dcgs2(int j) {
+    double* Vb = dptr<double>(V);
+    double* u = Vb + (int64_t)j * ld;
+    double* w = Vb + (int64_t)(j + 1) * ld;
+    const int m = j + 1;
+    const int g = std::min<int>(RED_BLOCKS, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
+    double* part = dptr<double>(partial);
+    double* part2 = part + (int64_t)RED_BLOCKS * MAXM;
+    const dim3 gd(g, (m + MDOT2_CHUNK - 1) / MDOT2_CHUNK);
+    hipLaunchKernelGGL((k_mdot2_partial<MDOT2_CHUNK>), gd, dim3(BLOCK), 0, st, n, m, (const double*)Vb, ld,
+                       (const double*)u, (const double*)w, part, part2);
+    hipLaunchKernelGGL(k_mdot2_final, dim3(2 * m), dim3(BLOCK), 0, st, g, m, (const double*)part,
+                       (const double*)part2, S(SC_H1), S(SC_H2));
+    hipLaunchKernelGGL(k_dcgs2_coef, dim3(1), dim3(1), 0, st, j, (const double*)S(SC_H1), (const double*)S(SC_H2),
+                       (const double*)S(SC_NRM), S(SC_RAW0), S(SC_RAW1), S(SC_DC));
+    hipLaunchKernelGGL(k_dcgs2_update, dim3(g), dim3(BLOCK), 0, st, n, j, Vb, ld, (const double*)S(SC_H1),
+                       (const double*)S(SC_H2), (const double*)S(SC_DC), w, part);
+    hipLaunchKernelGGL(k_givens_lag, dim3(1), dim3(BLOCK), 0, st, j, MAXM, g, (const double*)part,
+                       (const double*)S(SC_RAW0), (const double*)S(SC_RAW1), S(SC_HCOL), S(SC_H), S(SC_CS),
+                       S(SC_SN), S(SC_GAM), S(SC_DC) + 2, S(SC_NRM), S(SC_EST));
+    hipLaunchKernelGGL((k_scale_by_inv<double>), dim3(g), dim3(BLOCK), 0, st, n, w, w, S(SC_NRM), 0);
+    cnt.launches += 6;
+    cnt.bytes += 8.0 * n * ((m + 2.0) + (j + 4.0) + 2.0);
+  }
+
   void start_cycle(double* v0) {
     const int g = std::min<int>(4096, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
     hipLaunchKernelGGL((k_scale_by_inv<double>), dim3(g), dim3(BLOCK), 0, st, n, v0, v0, S(SC_NRM), 0);
@@ -570,6 +606,10 @@ struct fs_handle {
     const int m = restart_of(FS_METHOD_FGMRES);
     double* Vb = dptr<double>(V);
     T* Zb = (T*)Z.p;
+    // FS_FGMRES_CGS2=1: classical Gram-Schmidt twice per step instead of the
+    // delayed reorthogonalisation (for comparison)
+    const char* ge = std::getenv("FS_FGMRES_CGS2");
+    const bool cgs2 = ge && std::atoi(ge) == 1;
     int acc = 0, cycles = 0;
     bool conv = false;
     double est = 0;
@@ -594,7 +634,8 @@ struct fs_handle {
         // preconditioner is approximate, so the restarts' FP64 true residual
         // and the minimised residual belong to the same operator
         kop_t<double, T>(zj, w);
-        arnoldi_tail(j);
+        if (cgs2) arnoldi_tail(j);
+        else arnoldi_dcgs2(j);
         dim = j + 1;
         est = read_scalar(SC_EST);
         if (o.verbose > 1) std::fprintf(stderr, "  FGMRES step %d: estimate %.6e\n", acc, est);
@@ -1347,7 +1388,7 @@ static void create_handle(fs_handle& h, const fs_csr* A, const fs_csr* Bt, const
   h.pcg.alloc(8 * 6 * h.ldu);
   // z.s (or p.q) partials, then r.z partials (as many, or 512)
   h.pcg_part.alloc(8 * (size_t)(2 * h.spmv_dot_grid() + 1024));
-  h.partial.alloc(8 * RED_BLOCKS * MAXM);
+  h.partial.alloc(8 * RED_BLOCKS * MAXM * 2);  // two sets: the two-vector multi-dot
   h.sc.alloc(8 * SC_TOTAL);
   h.gbar.alloc(sizeof(GridBarrier));
   h.flags.alloc(64);
