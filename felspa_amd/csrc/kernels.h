@@ -13,7 +13,7 @@ constexpr int BLOCK = 256;      // threads per workgroup for streaming kernels
 constexpr int RED_BLOCKS = 512; // partial-sum slots of the two-stage reductions
 constexpr int MAXM = 256;       // max vectors in one multi-dot (restart + 1)
 constexpr int MDOT_CHUNK = 32;  // vectors per workgroup row of a multi-dot
-constexpr int MDOT2_CHUNK = 32; // basis vectors per workgroup row of the two-vector multi-dot
+constexpr int MDOT2_CHUNK = 16; // basis vectors per workgroup row of the two-vector multi-dot (32: 141 VGPRs, occupancy 3, C5 Krylov 70 ms slower)
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -613,35 +613,45 @@ static __global__ void k_mdot2_final(int nb, int m, const double* __restrict__ p
   if (threadIdx.x == 0) (second ? out2 : out)[k] = s;
 }
 
-// One thread: from a, alpha (h1[0..j]) and t, tau (h2[0..j]) the step's
-// 1 / r and v_j.w (dc[0], dc[1]); the previous column corrected into raw0
-// (rows 0..j, from the raw column j - 1 in raw1 and nu_{j-1} = nrm[0]) and
-// the new raw column j (rows 0..j) into raw1. u_0 = v_0 is final: r = 1.
-static __global__ void k_dcgs2_coef(int j, const double* __restrict__ h1, const double* __restrict__ h2,
-                                    const double* __restrict__ nrm, double* __restrict__ raw0,
-                                    double* __restrict__ raw1, double* __restrict__ dc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// From a, alpha (h1[0..j]) and t, tau (h2[0..j]) the step's 1 / r and
+// v_j.w (dc[0], dc[1]); the previous column corrected into raw0 (rows 0..j,
+// from the raw column j - 1 in raw1 and nu_{j-1} = nrm[0]) and the new raw
+// column j (rows 0..j) into raw1. u_0 = v_0 is final: r = 1. The workgroup
+// stages the inputs in LDS; thread 0 sums in index order.
+static __global__ __launch_bounds__(BLOCK) void k_dcgs2_coef(int j, const double* __restrict__ h1,
+                                                             const double* __restrict__ h2,
+                                                             const double* __restrict__ nrm,
+                                                             double* __restrict__ raw0, double* __restrict__ raw1,
+                                                             double* __restrict__ dc) {
+  __shared__ double la[MAXM], lt[MAXM], lr[MAXM];
+  for (int i = threadIdx.x; i <= j; i += blockDim.x) {
+    la[i] = h1[i];
+    lt[i] = h2[i];
+    lr[i] = raw1[i];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   if (j == 0) {
     dc[0] = 1.0;
-    dc[1] = h2[0];
-    raw1[0] = h2[0];
+    dc[1] = lt[0];
+    raw1[0] = lt[0];
     return;
   }
   double aa = 0, at = 0;
   for (int i = 0; i < j; ++i) {
-    aa += h1[i] * h1[i];
-    at += h1[i] * h2[i];
+    aa += la[i] * la[i];
+    at += la[i] * lt[i];
   }
   // r^2 <= 0: u_j lies in the span of the basis (a breakdown the projection
   // pass did not see); v_j = 0 then
-  const double r2 = h1[j] - aa;
+  const double r2 = la[j] - aa;
   const double r = r2 > 0.0 ? sqrt(r2) : 0.0;
   const double rinv = r > 0.0 ? 1.0 / r : 0.0;
-  const double hjj = (h2[j] - at) * rinv;
+  const double hjj = (lt[j] - at) * rinv;
   const double nu = nrm[0];
-  for (int i = 0; i < j; ++i) raw0[i] = raw1[i] + nu * h1[i];
+  for (int i = 0; i < j; ++i) raw0[i] = lr[i] + nu * la[i];
   raw0[j] = nu * r;
-  for (int i = 0; i < j; ++i) raw1[i] = h2[i];
+  for (int i = 0; i < j; ++i) raw1[i] = lt[i];
   raw1[j] = hjj;
   dc[0] = rinv;
   dc[1] = hjj;
@@ -701,43 +711,63 @@ static __global__ __launch_bounds__(BLOCK) void k_dcgs2_update(int64_t n, int j,
 // rotations 0..j-2, its own rotation, gamma from gsave = gamma_{j-1} before
 // that rotation) and makes column j provisional (raw1 rows 0..j and h_{j+1,j}
 // = nu_j, the norm before u_{j+1}'s reorthogonalisation), saving gamma_j
-// first; est = |gamma_{j+1}|. A cycle's last column stays provisional.
-static __global__ void k_givens_lag(int j, int ldh, int nb, const double* __restrict__ partial,
-                                    const double* __restrict__ raw0, const double* __restrict__ raw1,
-                                    double* __restrict__ hcol, double* __restrict__ H, double* __restrict__ cs,
-                                    double* __restrict__ sn, double* __restrict__ gam, double* __restrict__ gsave,
-                                    double* __restrict__ nrm, double* __restrict__ est) {
+// first; est = |gamma_{j+1}|. A cycle's last column stays provisional. The
+// rotations and raw columns are staged in LDS by the workgroup; the
+// rotation sweep carries the running element in a register (the same
+// arithmetic as k_givens).
+static __global__ __launch_bounds__(BLOCK) void k_givens_lag(int j, int ldh, int nb, const double* __restrict__ partial,
+                                                             const double* __restrict__ raw0,
+                                                             const double* __restrict__ raw1,
+                                                             double* __restrict__ H, double* __restrict__ cs,
+                                                             double* __restrict__ sn, double* __restrict__ gam,
+                                                             double* __restrict__ gsave, double* __restrict__ nrm,
+                                                             double* __restrict__ est) {
   __shared__ double sh[BLOCK / WAVE];
+  __shared__ double lc[MAXM], ls[MAXM], c0[MAXM + 1], c1[MAXM + 2];
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    lc[i] = cs[i];
+    ls[i] = sn[i];
+  }
+  for (int i = threadIdx.x; i <= j; i += blockDim.x) {
+    c0[i] = raw0[i];
+    c1[i] = raw1[i];
+  }
   double s = 0;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
-  s = block_sum(s, sh);
+  s = block_sum(s, sh);  // (its barriers also publish the staged values)
   if (threadIdx.x != 0) return;
   const double nu = sqrt(s);
   nrm[0] = nu;
-  auto rotate = [&](int c) {  // column c in hcol[0..c+1]
+  auto rotate = [&](double* col, int c) {  // column c in col[0..c+1]
+    double carry = col[0];
     for (int i = 0; i < c; ++i) {
-      const double hi = hcol[i];
-      hcol[i] = cs[i] * hi + sn[i] * hcol[i + 1];
-      hcol[i + 1] = -sn[i] * hi + cs[i] * hcol[i + 1];
+      const double nx = col[i + 1];
+      const double ci = lc[i], si = ls[i];
+      col[i] = ci * carry + si * nx;
+      carry = -si * carry + ci * nx;
     }
-    const double r = 1.0 / sqrt(hcol[c] * hcol[c] + hcol[c + 1] * hcol[c + 1]);
-    sn[c] = hcol[c + 1] * r;
-    cs[c] = hcol[c] * r;
-    hcol[c] = cs[c] * hcol[c] + sn[c] * hcol[c + 1];
-    for (int i = 0; i <= c; ++i) H[(int64_t)i * ldh + c] = hcol[i];
+    const double b = col[c + 1];
+    const double r = 1.0 / sqrt(carry * carry + b * b);
+    const double sc = b * r, cc = carry * r;
+    ls[c] = sc;
+    lc[c] = cc;
+    sn[c] = sc;
+    cs[c] = cc;
+    col[c] = cc * carry + sc * b;
+    for (int i = 0; i <= c; ++i) H[(int64_t)i * ldh + c] = col[i];
   };
   if (j > 0) {
-    for (int i = 0; i <= j; ++i) hcol[i] = raw0[i];
-    rotate(j - 1);
-    gam[j] = -sn[j - 1] * gsave[0];
-    gam[j - 1] = cs[j - 1] * gsave[0];
+    rotate(c0, j - 1);
+    const double g = gsave[0];
+    gam[j] = -ls[j - 1] * g;
+    gam[j - 1] = lc[j - 1] * g;
   }
-  for (int i = 0; i <= j; ++i) hcol[i] = raw1[i];
-  hcol[j + 1] = nu;
-  rotate(j);
-  gsave[0] = gam[j];
-  gam[j + 1] = -sn[j] * gam[j];
-  gam[j] *= cs[j];
+  c1[j + 1] = nu;
+  rotate(c1, j);
+  const double gj = gam[j];
+  gsave[0] = gj;
+  gam[j + 1] = -ls[j] * gj;
+  gam[j] = gj * lc[j];
   est[0] = fabs(gam[j + 1]);
 }
 

@@ -563,13 +563,13 @@ struct fs_handle {
                        (const double*)u, (const double*)w, part, part2);
     hipLaunchKernelGGL(k_mdot2_final, dim3(2 * m), dim3(BLOCK), 0, st, g, m, (const double*)part,
                        (const double*)part2, S(SC_H1), S(SC_H2));
-    hipLaunchKernelGGL(k_dcgs2_coef, dim3(1), dim3(1), 0, st, j, (const double*)S(SC_H1), (const double*)S(SC_H2),
+    hipLaunchKernelGGL(k_dcgs2_coef, dim3(1), dim3(BLOCK), 0, st, j, (const double*)S(SC_H1), (const double*)S(SC_H2),
                        (const double*)S(SC_NRM), S(SC_RAW0), S(SC_RAW1), S(SC_DC));
     hipLaunchKernelGGL(k_dcgs2_update, dim3(g), dim3(BLOCK), 0, st, n, j, Vb, ld, (const double*)S(SC_H1),
                        (const double*)S(SC_H2), (const double*)S(SC_DC), w, part);
     hipLaunchKernelGGL(k_givens_lag, dim3(1), dim3(BLOCK), 0, st, j, MAXM, g, (const double*)part,
-                       (const double*)S(SC_RAW0), (const double*)S(SC_RAW1), S(SC_HCOL), S(SC_H), S(SC_CS),
-                       S(SC_SN), S(SC_GAM), S(SC_DC) + 2, S(SC_NRM), S(SC_EST));
+                       (const double*)S(SC_RAW0), (const double*)S(SC_RAW1), S(SC_H), S(SC_CS), S(SC_SN),
+                       S(SC_GAM), S(SC_DC) + 2, S(SC_NRM), S(SC_EST));
     hipLaunchKernelGGL((k_scale_by_inv<double>), dim3(g), dim3(BLOCK), 0, st, n, w, w, S(SC_NRM), 0);
     cnt.launches += 6;
     cnt.bytes += 8.0 * n * ((m + 2.0) + (j + 4.0) + 2.0);
