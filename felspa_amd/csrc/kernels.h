@@ -1605,6 +1605,67 @@ static __global__ __launch_bounds__(BLOCK) void k_ilu_block_inverse(IluView<doub
   }
 }
 
+// The same inverses with the block in LDS: the in-block L and U entries
+// scattered into a dense nr x nr tile D (row-major, even leading dimension
+// ld), then thread j forms column j of L_bb^{-1} in the tile X (forward
+// substitution; the sum over k ascending, as above), the workgroup writes
+// X's columns to the packed output, and likewise U_bb^{-1} (backward
+// substitution). Lane j reads element (j + t) (ld + 1) + c of D and X:
+// consecutive lanes an odd number of elements apart, no bank conflicts.
+// LDS: 2 nr_max ld doubles.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ilu_block_inverse_lds(IluView<double> F, int nblk,
+                                                              const int64_t* __restrict__ inv_off,
+                                                              double* __restrict__ linv, double* __restrict__ uinv,
+                                                              int ld) {
+  extern __shared__ __attribute__((aligned(16))) char ism[];
+  double* D = (double*)ism;
+  const int tid = threadIdx.x, lane = tid % WAVE, wv = tid / WAVE;
+  constexpr int NW = NT / WAVE;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int r0 = F.blk_ptr[blk], nr = F.blk_ptr[blk + 1] - r0;
+    double* X = D + (size_t)nr * ld;
+    for (int e = tid; e < nr * ld; e += NT) D[e] = 0.0;
+    __syncthreads();
+    for (int i = wv; i < nr; i += NW) {
+      const int r = r0 + i;
+      const int64_t base = F.rp[r];
+      for (int p = F.o_lint[r] + lane; p < F.o_uext[r]; p += WAVE) D[i * ld + (F.ci[base + p] - r0)] = F.v[base + p];
+    }
+    __syncthreads();
+    for (int j = tid; j < nr; j += NT) {
+      X[j * ld + j] = 1.0;
+      for (int i = j + 1; i < nr; ++i) {
+        double s = 0;
+        for (int k = j; k < i; ++k) s += D[i * ld + k] * X[k * ld + j];
+        X[i * ld + j] = -s;
+      }
+    }
+    __syncthreads();
+    double* Lb = linv + inv_off[blk];
+    for (int j = wv; j < nr; j += NW) {
+      double* col = Lb + (int64_t)j * nr - (int64_t)j * (j - 1) / 2;  // rows j..nr-1
+      for (int i = j + lane; i < nr; i += WAVE) col[i - j] = X[i * ld + j];
+    }
+    __syncthreads();
+    for (int j = tid; j < nr; j += NT) {
+      X[j * ld + j] = 1.0 / D[j * ld + j];
+      for (int i = j - 1; i >= 0; --i) {
+        double s = 0;
+        for (int k = i + 1; k <= j; ++k) s += D[i * ld + k] * X[k * ld + j];
+        X[i * ld + j] = -s / D[i * ld + i];
+      }
+    }
+    __syncthreads();
+    double* Ub = uinv + inv_off[blk];
+    for (int j = wv; j < nr; j += NW) {
+      double* col = Ub + (int64_t)j * (j + 1) / 2;  // rows 0..j
+      for (int i = lane; i <= j; i += WAVE) col[i] = X[i * ld + j];
+    }
+    __syncthreads();  // X and D are rewritten for the next block
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Numeric ILU (include/felspa/linear_algebra/src/ilu.implement.h:438-541):
 // each workgroup of NT threads factorises the rows of one block in order.

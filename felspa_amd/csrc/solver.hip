@@ -789,8 +789,20 @@ void IluDev::factorize(const int64_t* srp, const int32_t* sci, const double* sv,
       hipLaunchKernelGGL(k_gather_ext, dim3(gb), dim3(BLOCK), 0, st, xudesc.as<i32x4>(), xurng.as<int32_t>(),
                          rp.as<int64_t>(), o_lint.as<int32_t>(), o_uext.as<int32_t>(), 0, v64.as<double>(),
                          xuv64.as<double>(), xuv32.as<float>(), nb);
-    hipLaunchKernelGGL(k_ilu_block_inverse, dim3(std::max(1, std::min(nb, 65535))), dim3(BLOCK), 0, st, V, nb,
-                       inv_off.as<int64_t>(), linv64.as<double>(), uinv64.as<double>());
+    // the block in LDS as two dense tiles when they fit (at most 99 rows)
+    const int ldi = (std::max(1, (int)sched.max_block_rows) + 1) & ~1;
+    const size_t ldsi = (size_t)2 * sched.max_block_rows * ldi * sizeof(double);
+    if (ldsi <= dense_wg_lds_limit()) {
+      auto fn = &k_ilu_block_inverse_lds<256>;
+      if (ldsi > 64 * 1024)
+        hip_check(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsi),
+                  "hipFuncSetAttribute");
+      hipLaunchKernelGGL(fn, dim3(std::max(1, std::min(nb, 65535))), dim3(256), ldsi, st, V, nb, inv_off.as<int64_t>(),
+                         linv64.as<double>(), uinv64.as<double>(), ldi);
+    } else {
+      hipLaunchKernelGGL(k_ilu_block_inverse, dim3(std::max(1, std::min(nb, 65535))), dim3(BLOCK), 0, st, V, nb,
+                         inv_off.as<int64_t>(), linv64.as<double>(), uinv64.as<double>());
+    }
     hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(inv_total)), dim3(BLOCK), 0, st, inv_total,
                        linv64.as<double>(), linv32.as<float>());
     hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(inv_total)), dim3(BLOCK), 0, st, inv_total,
