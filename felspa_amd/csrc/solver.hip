@@ -303,18 +303,29 @@ void IluDev::build(const HostPattern& src_pattern_new, const Schedule& sc, int k
         NumBin nb;
         nb.mcap = caps[c];
         nb.G = g;
-        // hash load factor <= 1/4 unless that costs workgroups per CU,
-        // never above 1/2; groups that do not fit split into single rows
+        // the map is keyed by node: a row names at most mcap nodes (usually
+        // a third as many), so mcap + 1 slots always leave one free; 4 mcap
+        // unless that costs workgroups per CU. Triples that do not fit split
+        // into a pair and a single row (same columns), else into single rows
         int hmin = 64, hgood = 64;
-        while (hmin < 2 * nb.mcap) hmin *= 2;
+        while (hmin < nb.mcap + 1) hmin *= 2;
         while (hgood < 4 * nb.mcap) hgood *= 2;
         const size_t wmin = ilu_update_lds_bytes(nb.mcap, hmin, g), wgood = ilu_update_lds_bytes(nb.mcap, hgood, g);
         if (wmin > lim) {
           if (g == 1) throw FsError(FS_ERR_UNSUPPORTED, "ILU row too long for the stage update's LDS");
+          const bool two = g == 3 && ilu_update_lds_bytes(nb.mcap, hmin, 2) <= lim;
           auto& one = lists[3 * c];
-          for (int32_t r0 : lst)
-            for (int k2 = 0; k2 < g; ++k2) one.push_back(r0 + k2);
+          auto& pair = lists[3 * c + 1];
+          for (int32_t r0 : lst) {
+            if (two) {
+              pair.push_back(r0);
+              one.push_back(r0 + 2);
+            } else {
+              for (int k2 = 0; k2 < g; ++k2) one.push_back(r0 + k2);
+            }
+          }
           std::sort(one.begin(), one.end());
+          std::sort(pair.begin(), pair.end());
           continue;
         }
         nb.hs = (wgood <= lim && lim / wgood >= lim / wmin) ? hgood : hmin;
