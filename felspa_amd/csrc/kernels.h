@@ -13,7 +13,6 @@ constexpr int BLOCK = 256;      // threads per workgroup for streaming kernels
 constexpr int RED_BLOCKS = 512; // partial-sum slots of the two-stage reductions
 constexpr int MAXM = 256;       // max vectors in one multi-dot (restart + 1)
 constexpr int MDOT_CHUNK = 32;  // vectors per workgroup row of a multi-dot
-constexpr int MDOT2_CHUNK = 16; // basis vectors per workgroup row of the two-vector multi-dot (32: 141 VGPRs, occupancy 3, C5 Krylov 70 ms slower)
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -551,50 +550,68 @@ __global__ __launch_bounds__(BLOCK) void k_gs_update(int64_t n, int m, const dou
 // projected w. The previous Hessenberg column gets nu_{j-1} a added and
 // h_{j,j-1} = nu_{j-1} r.
 
-// partial[b*MAXM + k] = V_k . x1, partial2[b*MAXM + k] = V_k . x2 (k < m):
-// k_mdot_partial for two vectors, V read once
-template <int CH>
-__global__ __launch_bounds__(BLOCK) void k_mdot2_partial(int64_t n, int m, const double* __restrict__ V, int64_t ld,
-                                                         const double* __restrict__ x1,
-                                                         const double* __restrict__ x2,
-                                                         double* __restrict__ partial,
-                                                         double* __restrict__ partial2) {
-  __shared__ double sh[2 * CH][BLOCK / WAVE];
-  const int k0 = blockIdx.y * CH;
-  const int mc = min(CH, m - k0);
-  const double* Vc = V + (int64_t)k0 * ld;
-  double a1[CH], a2[CH];
-#pragma unroll
-  for (int k = 0; k < CH; ++k) a1[k] = a2[k] = 0;
-  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-    const double y1 = x1[i], y2 = x2[i];
-#pragma unroll
-    for (int k = 0; k < CH; ++k)
-      if (k < mc) {
-        const double v = Vc[(int64_t)k * ld + i];
-        a1[k] += v * y1;
-        a2[k] += v * y2;
-      }
+// partial[b*MAXM + k] = V_k . x1, partial2[b*MAXM + k] = V_k . x2 (k < m),
+// by row tiles: a workgroup takes tiles of BLOCK R rows (grid-stride),
+// holds its rows of x1 and x2 in registers and walks the
+// m basis vectors U at a time (U R loads in flight per thread), so V, x1
+// and x2 are each read once; per vector every wave sums its lanes and keeps
+// the sum in LDS, and the waves' sums are added in wave order at the end.
+template <int R, int U>
+__global__ __launch_bounds__(BLOCK) void k_mdot2_rows(int64_t n, int m, const double* __restrict__ V, int64_t ld,
+                                                      const double* __restrict__ x1, const double* __restrict__ x2,
+                                                      double* __restrict__ partial, double* __restrict__ partial2) {
+  constexpr int NW = BLOCK / WAVE;
+  __shared__ double s1[NW][MAXM], s2[NW][MAXM];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  for (int k = lane; k < m; k += WAVE) {  // each wave its own row (in order within the wave)
+    s1[wv][k] = 0;
+    s2[wv][k] = 0;
   }
-  const int wv = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+  for (int64_t i0 = (int64_t)blockIdx.x * BLOCK * R; i0 < n; i0 += (int64_t)gridDim.x * BLOCK * R) {
+    double a1[R], a2[R];
+    int64_t ri[R];
 #pragma unroll
-  for (int k = 0; k < CH; ++k) {
-    if (k < mc) {
-      const double s1 = group_reduce<WAVE>(a1[k]);
-      const double s2 = group_reduce<WAVE>(a2[k]);
-      if (l == 0) {
-        sh[k][wv] = s1;
-        sh[CH + k][wv] = s2;
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = i0 + (int64_t)r * BLOCK + threadIdx.x;
+      ri[r] = i < n ? i : 0;  // past the end: row 0 times zero weights
+      a1[r] = i < n ? x1[i] : 0.0;
+      a2[r] = i < n ? x2[i] : 0.0;
+    }
+    for (int k0 = 0; k0 < m; k0 += U) {
+      double v[U][R];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t col = (int64_t)(k0 + u < m ? k0 + u : m - 1) * ld;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u][r] = V[col + ri[r]];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double p1 = 0, p2 = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          p1 += v[u][r] * a1[r];
+          p2 += v[u][r] * a2[r];
+        }
+        p1 = group_reduce<WAVE>(p1);
+        p2 = group_reduce<WAVE>(p2);
+        if (lane == 0 && k0 + u < m) {
+          s1[wv][k0 + u] += p1;
+          s2[wv][k0 + u] += p2;
+        }
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x < (unsigned)(2 * mc)) {
-    const int k = threadIdx.x < (unsigned)mc ? threadIdx.x : threadIdx.x - mc;
-    const int row = threadIdx.x < (unsigned)mc ? k : CH + k;
-    double s = 0;
-    for (int i = 0; i < BLOCK / WAVE; ++i) s += sh[row][i];
-    (threadIdx.x < (unsigned)mc ? partial : partial2)[(int64_t)blockIdx.x * MAXM + k0 + k] = s;
+  for (int k = threadIdx.x; k < m; k += BLOCK) {
+    double t1 = 0, t2 = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      t1 += s1[w][k];
+      t2 += s2[w][k];
+    }
+    partial[(int64_t)blockIdx.x * MAXM + k] = t1;
+    partial2[(int64_t)blockIdx.x * MAXM + k] = t2;
   }
 }
 

@@ -558,8 +558,10 @@ struct fs_handle {
     const int g = std::min<int>(RED_BLOCKS, (int)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK));
     double* part = dptr<double>(partial);
     double* part2 = part + (int64_t)RED_BLOCKS * MAXM;
-    const dim3 gd(g, (m + MDOT2_CHUNK - 1) / MDOT2_CHUNK);
-    hipLaunchKernelGGL((k_mdot2_partial<MDOT2_CHUNK>), gd, dim3(BLOCK), 0, st, n, m, (const double*)Vb, ld,
+    // row tiles of 8 rows per thread, 2 basis vectors per round (16 loads in
+    // flight; 4 x 4 measured the same; 16-vector chunks re-reading x1 and x2
+    // per chunk took 0.29 ms more per step at C5)
+    hipLaunchKernelGGL((k_mdot2_rows<8, 2>), dim3(g), dim3(BLOCK), 0, st, n, m, (const double*)Vb, ld,
                        (const double*)u, (const double*)w, part, part2);
     hipLaunchKernelGGL(k_mdot2_final, dim3(2 * m), dim3(BLOCK), 0, st, g, m, (const double*)part,
                        (const double*)part2, S(SC_H1), S(SC_H2));
