@@ -2168,18 +2168,21 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
             m1[g] = fma(-m0[g], u01, w[g * ld + t + 1]) * r1;
             m2[g] = fma(-m1[g], u12, fma(-m0[g], u02, w[g * ld + t + 2])) * r2;
           }
+          if (tid == 0) {  // one branch: the step's multipliers go to the factor
 #pragma unroll
-          for (int g = 0; g < G; ++g) {
-            if (tid == 3 * g) F.v[bg[g] + t] = m0[g];
-            if (tid == 3 * g + 1) F.v[bg[g] + t + 1] = m1[g];
-            if (tid == 3 * g + 2) F.v[bg[g] + t + 2] = m2[g];
+            for (int g = 0; g < G; ++g) {
+              F.v[bg[g] + t] = m0[g];
+              F.v[bg[g] + t + 1] = m1[g];
+              F.v[bg[g] + t + 2] = m2[g];
+            }
           }
         } else {
           const double rp = mu[sl];
 #pragma unroll
-          for (int g = 0; g < G; ++g) {
-            m0[g] = w[g * ld + t] * rp;
-            if (tid == g) F.v[bg[g] + t] = m0[g];
+          for (int g = 0; g < G; ++g) m0[g] = w[g * ld + t] * rp;
+          if (tid == 0) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) F.v[bg[g] + t] = m0[g];
           }
         }
       }
