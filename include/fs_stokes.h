@@ -79,11 +79,13 @@ typedef struct fs_opts {
   int32_t max_outer;     /* 0: n_u + n_p (stokes_ilu.cc:534) */
   double rtol;           /* tol = rtol ||rhs|| (stokes_ilu.cc:533), default 1e-8 */
   double reduce;         /* ReductionControl reduce (stokes_common.h:343-348), default 1e-6 */
-  int32_t inner_cg_max;  /* 0: n_p (linear_operators.implement.h:76) */
+  int32_t inner_cg_max;  /* 0: n_p (linear_operators.implement.h:76); 1 (a single ILU-PCG step,
+                            the failure flag set when it misses the tolerance) runs as ordinary
+                            kernel launches, more steps as one cooperative kernel */
   double inner_cg_tol;   /* InverseMatrixCG tol = reduce (linear_operators.h:44-45), default 1e-12;
                             FS_METHOD_FGMRES in FP64 uses it as a relative reduction only */
   double inner_cg_tol_mixed; /* FP32 inner CG relative tolerance in FS_PREC_MIXED */
-  int32_t inner_cg_max_mixed;
+  int32_t inner_cg_max_mixed; /* its step cap (1: as for inner_cg_max) */
   int32_t apply_diagonal_scaling; /* StokesSolverControl::apply_diagonal_scaling (stokes_common.h:360) */
   int32_t guess;         /* FS_GUESS_* */
   int32_t block_rows;    /* target rows per block for FS_ORDER_BLOCK_COLOR (0: default) */
