@@ -2001,12 +2001,12 @@ __device__ __forceinline__ int node_probe(const uint32_t* tab, const int32_t* lc
 
 // LDS of one stage-update workgroup: the G rows' working values from the
 // first updated column on (and a dump slot each), a batch of 64 L columns'
-// U spans, pivots, first two U values, node-triple flags and steps, the
-// columns and the hash map
+// U spans, pivots, first two U values, node-triple flags, the steps with
+// their streams' starts and lengths, the columns and the hash map
 constexpr int UPD_BATCH = 64;
 __host__ __device__ inline size_t ilu_update_lds_bytes(int mcap, int hs, int G) {
-  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 8 + 8 + 4 + 4 + 4) + 16 + (size_t)mcap * 4 +
-         (size_t)hs * 4;
+  return (size_t)G * (mcap + 1) * 8 + (size_t)UPD_BATCH * (8 + 8 + 8 + 8 + 24 + 4 + 4 + 4 + 4) + 16 +
+         (size_t)mcap * 4 + (size_t)hs * 4;
 }
 
 // One stage update: the row groups starting at rows[0 .. nrows) (G
@@ -2043,10 +2043,12 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
   double* mu = (double*)(mq + UPD_BATCH);         // [UPD_BATCH] 1 / u_kk
   double* mf = mu + UPD_BATCH;                    // [UPD_BATCH] first U value of row k
   double* ms = mf + UPD_BATCH;                    // [UPD_BATCH] second U value of row k
-  int32_t* mn = (int32_t*)(ms + UPD_BATCH);       // [UPD_BATCH] U entries of row k
+  int64_t* sq = (int64_t*)(ms + UPD_BATCH);       // [3][UPD_BATCH] per step: its three U streams
+  int32_t* mn = (int32_t*)(sq + 3 * UPD_BATCH);   // [UPD_BATCH] U entries of row k
   int32_t* tf = mn + UPD_BATCH;                   // [UPD_BATCH] slot starts a node triple
   int32_t* stp = tf + UPD_BATCH;                  // [UPD_BATCH] steps: slot | width << 8
-  int32_t* nsteps = stp + UPD_BATCH;              // [4] the batch's step count
+  int32_t* sn = stp + UPD_BATCH;                  // [UPD_BATCH] per step: its stream's length
+  int32_t* nsteps = sn + UPD_BATCH;               // [4] the batch's step count
   int32_t* lc = nsteps + 4;                       // [mcap] columns
   uint32_t* tab = (uint32_t*)(lc + mcap);         // [hs] hash map
   const uint32_t mask = (uint32_t)hs - 1;
@@ -2101,7 +2103,19 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
       const bool mem = (lane >= 1 && lane - 1 < nb && tf[lane - 1] != 0) || (lane >= 2 && lane - 2 < nb && tf[lane - 2] != 0);
       const bool is_step = lane < nb && !mem;
       const uint64_t bal = __ballot(is_step);
-      if (is_step) stp[__popcll(bal & ((1ull << lane) - 1ull))] = lane | ((st0 ? 3 : 1) << 8);
+      if (is_step) {
+        // the step's record: its slot and width, the starts of the three
+        // streams the chunks load (a single column reads row k's U part three
+        // times, the extra values unused; a triple reads row k past k+1, k+2,
+        // row k+1 past k+2 and row k+2's U part, which name the same columns)
+        // and the stream's length
+        const int c = __popcll(bal & ((1ull << lane) - 1ull));
+        stp[c] = lane | ((st0 ? 3 : 1) << 8);
+        sq[c] = st0 ? mq[lane] + 2 : mq[lane];
+        sq[UPD_BATCH + c] = st0 ? mq[lane + 1] + 1 : mq[lane];
+        sq[2 * UPD_BATCH + c] = st0 ? mq[lane + 2] : mq[lane];
+        sn[c] = st0 ? mn[lane + 2] : mn[lane];
+      }
       if (lane == 0) nsteps[0] = __popcll(bal);
     }
     lds_barrier();
@@ -2115,11 +2129,7 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
     // chunk issues the loads of a triple, so the loads in flight are the same
     // on every path.
     constexpr int CHUNK = NT * M;
-    auto slen = [&](int c) {
-      const int sd = stp[c];
-      const int sl = sd & 0xff;
-      return (sd >> 8) == 3 ? mn[sl + 2] : mn[sl];
-    };
+    auto slen = [&](int c) { return sn[c]; };
     auto adv = [&](int2 p) -> int2 {
       if (p.x >= ns) return p;
       p.y += CHUNK;
@@ -2130,15 +2140,11 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
       return p;
     };
     auto issue = [&](int2 p, int32_t* j, double (*v)[M]) {
+      // the step's record (past the batch: step 0's, with no entries)
       const bool ok = p.x < ns;
-      const int sd = ok ? stp[p.x] : 0;
-      const int sl = sd & 0xff;
-      const bool t3 = ok && (sd >> 8) == 3;
-      // the stream's rows: a single column reads row k's U part three times
-      // (the extra values are not used); a triple reads row k past k+1, k+2,
-      // row k+1 past k+2 and row k+2's U part, which name the same columns
-      const int64_t q0 = t3 ? mq[sl] + 2 : mq[sl], q1 = t3 ? mq[sl + 1] + 1 : mq[sl], q2 = t3 ? mq[sl + 2] : mq[sl];
-      const int n = ok ? (t3 ? mn[sl + 2] : mn[sl]) : 0;
+      const int c = ok ? p.x : 0;
+      const int64_t q0 = sq[c], q1 = sq[UPD_BATCH + c], q2 = sq[2 * UPD_BATCH + c];
+      const int n = ok ? sn[c] : 0;
 #pragma unroll
       for (int r = 0; r < M; ++r) {
         const int idx = p.y + tid + r * NT;
@@ -2186,7 +2192,7 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
           }
         }
       }
-      const int n = t3 ? mn[sl + 2] : mn[sl];
+      const int n = sn[p.x];
       uint32_t q[M], h[M], e[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
