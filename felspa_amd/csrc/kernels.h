@@ -2129,17 +2129,18 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
     // chunk issues the loads of a triple, so the loads in flight are the same
     // on every path.
     constexpr int CHUNK = NT * M;
-    auto slen = [&](int c) { return sn[c]; };
-    auto adv = [&](int2 p) -> int2 {
+    // chunk p of a stream of n entries -> the next chunk (n as issue()
+    // returned it for p)
+    auto adv = [&](int2 p, int n) -> int2 {
       if (p.x >= ns) return p;
       p.y += CHUNK;
-      if (p.y >= slen(p.x)) {
+      if (p.y >= n) {
         p.x += 1;
         p.y = 0;
       }
       return p;
     };
-    auto issue = [&](int2 p, int32_t* j, double (*v)[M]) {
+    auto issue = [&](int2 p, int32_t* j, double (*v)[M]) -> int {
       // the step's record (past the batch: step 0's, with no entries)
       const bool ok = p.x < ns;
       const int c = ok ? p.x : 0;
@@ -2154,11 +2155,12 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
         v[1][r] = F.v[in ? q1 + idx : 0];
         v[2][r] = F.v[in ? q2 + idx : 0];
       }
+      return n;
     };
     double m0[G], m1[G], m2[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) m0[g] = m1[g] = m2[g] = 0.0;
-    auto process = [&](int2 p, const int32_t* j, double (*v)[M]) {
+    auto process = [&](int2 p, const int32_t* j, double (*v)[M], int n) {
       const int sd = stp[p.x];
       const int sl = sd & 0xff;
       const bool t3 = (sd >> 8) == 3;
@@ -2192,7 +2194,6 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
           }
         }
       }
-      const int n = sn[p.x];
       uint32_t q[M], h[M], e[M];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
@@ -2229,10 +2230,11 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
       }
     };
     int2 p0 = make_int2(0, 0);
-    int2 p1 = adv(p0), p2 = adv(p1);
-    issue(p0, ja, va);
-    issue(p1, jb, vb);
-    issue(p2, jc, vc);
+    int na = issue(p0, ja, va);
+    int2 p1 = adv(p0, na);
+    int nb1 = issue(p1, jb, vb);
+    int2 p2 = adv(p1, nb1);
+    int nc1 = issue(p2, jc, vc);
     // One exit, at the loop test: every pass issues the three chunks' loads
     // (past the batch, loads of entry 0), so the loads in flight when a
     // chunk is used are the same on every path into it and the compiler's
@@ -2241,15 +2243,15 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
     // Positions only advance, so a chunk past the batch is followed by
     // chunks past it.
     while (p0.x < ns) {
-      process(p0, ja, va);
-      const int2 p3 = adv(p2);
-      issue(p3, ja, va);
-      if (p1.x < ns) process(p1, jb, vb);
-      const int2 p4 = adv(p3);
-      issue(p4, jb, vb);
-      if (p2.x < ns) process(p2, jc, vc);
-      const int2 p5 = adv(p4);
-      issue(p5, jc, vc);
+      process(p0, ja, va, na);
+      const int2 p3 = adv(p2, nc1);
+      na = issue(p3, ja, va);
+      if (p1.x < ns) process(p1, jb, vb, nb1);
+      const int2 p4 = adv(p3, na);
+      nb1 = issue(p4, jb, vb);
+      if (p2.x < ns) process(p2, jc, vc, nc1);
+      const int2 p5 = adv(p4, nb1);
+      nc1 = issue(p5, jc, vc);
       p0 = p3;
       p1 = p4;
       p2 = p5;
