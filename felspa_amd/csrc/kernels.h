@@ -2129,12 +2129,13 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
     // chunk issues the loads of a triple, so the loads in flight are the same
     // on every path.
     constexpr int CHUNK = NT * M;
-    // chunk p of a stream of n entries -> the next chunk (n as issue()
-    // returned it for p)
-    auto adv = [&](int2 p, int n) -> int2 {
+    // chunk p -> the next chunk; ln as issue() returned it for p: the
+    // stream's length (low 16 bits, at most 8192) and the step's slot and
+    // width (high 16 bits)
+    auto adv = [&](int2 p, int ln) -> int2 {
       if (p.x >= ns) return p;
       p.y += CHUNK;
-      if (p.y >= n) {
+      if (p.y >= (ln & 0xffff)) {
         p.x += 1;
         p.y = 0;
       }
@@ -2146,6 +2147,7 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
       const int c = ok ? p.x : 0;
       const int64_t q0 = sq[c], q1 = sq[UPD_BATCH + c], q2 = sq[2 * UPD_BATCH + c];
       const int n = ok ? sn[c] : 0;
+      const int sd = stp[c];
 #pragma unroll
       for (int r = 0; r < M; ++r) {
         const int idx = p.y + tid + r * NT;
@@ -2155,13 +2157,13 @@ __device__ __forceinline__ void ilu_group_columns(const IluView<double>& F, cons
         v[1][r] = F.v[in ? q1 + idx : 0];
         v[2][r] = F.v[in ? q2 + idx : 0];
       }
-      return n;
+      return n | (sd << 16);
     };
     double m0[G], m1[G], m2[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) m0[g] = m1[g] = m2[g] = 0.0;
-    auto process = [&](int2 p, const int32_t* j, double (*v)[M], int n) {
-      const int sd = stp[p.x];
+    auto process = [&](int2 p, const int32_t* j, double (*v)[M], int ln) {
+      const int n = ln & 0xffff, sd = ln >> 16;
       const int sl = sd & 0xff;
       const bool t3 = (sd >> 8) == 3;
       if (p.y == 0) {  // a new step: every earlier step's updates are in
